@@ -1,0 +1,204 @@
+/*
+ * staccato_sweep.h -- C ABI of the MI355X frequency-sweep library
+ * (libstaccato_sweep.so, built from staccato_amd/csrc/).
+ *
+ * For every excitation frequency f (omega = 2 pi f) the library forms the
+ * reduced dynamic stiffness
+ *
+ *     A(omega) = (1 + i eta) K + i omega D - omega^2 M
+ *
+ * from the ROM operators K, M, D and solves A(omega) X = F by LU with partial
+ * pivoting, entirely on one AMD Instinct MI355X (gfx950).  A(omega) is
+ * assembled inside the factorisation kernels and never stored per frequency.
+ *
+ * What this replaces in the reference (DrStS/STACCATO):
+ *   - the body of the frequency loop of KrylovROMSubstructure::backTransformKMOR
+ *     (src/analysis/KrylovROMSubstructure.cpp:1403-1431: three cblas_zaxpy
+ *     assembly calls, LAPACKE_zgetrf, LAPACKE_zgetrs),
+ *   - the dense-mode frequency loop of the benchmark
+ *     (srcPerformanceTest/main.cpp:375-402 with
+ *      srcPerformanceTest/solver/assembly.cpp:8-12),
+ *   - the cuBLAS getrfBatched / cuSOLVER getrf tests
+ *     (test/krylovMidsizeTest/gpu/concurrent/batchedLU/src/main.cu:248-259,
+ *      test/krylovTestGPU/matrixInSeparateLoop/gpu/src/main.cu:205-222),
+ *     which factor without pivoting; this library always pivots.
+ *
+ * Conventions (as in the reference):
+ *   - column-major, leading dimensions given explicitly;
+ *   - complex numbers interleaved {double re, im}: stc_z64 is bit-identical
+ *     to MKL_Complex16 = STACCATOComplexDouble
+ *     (src/tools/AuxiliaryParameters.h:52);
+ *   - pivots are 1-based 32-bit ints with LAPACK zgetrf meaning
+ *     (LP64 lapack_int, cmake/modules/FindMKL.cmake:57-72);
+ *   - per-frequency info: 0 = solved, k > 0 = U(k,k) is exactly zero
+ *     (LAPACK zgetrf info), -1 = iterative refinement did not reach tol.
+ *     The reference ignores zgetrf's info (KrylovROMSubstructure.cpp:1428);
+ *     this library reports it and never calls exit().
+ *
+ * Threading: a handle owns one device, one HIP stream and its device buffers.
+ * Use one handle per host thread; calls on one handle are not thread safe.
+ * Calls taking host pointers are synchronous.  The *_dev calls take device
+ * pointers and a stream (a hipStream_t passed as void*, NULL = the handle's
+ * stream) and are asynchronous with respect to the host.
+ */
+#ifndef STACCATO_SWEEP_H
+#define STACCATO_SWEEP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } stc_z64; /* == MKL_Complex16 */
+typedef struct { float re, im; } stc_c32;  /* == MKL_Complex8  */
+
+typedef struct stc_ctx* stc_handle;
+typedef struct stc_plan_s* stc_plan;
+
+/* status codes returned by every call */
+enum {
+    STC_OK = 0,
+    STC_EINVAL = -1,   /* bad argument (size, pointer, rom id, mode)      */
+    STC_ENOMEM = -2,   /* device allocation failed                        */
+    STC_EDEVICE = -3,  /* HIP runtime error                               */
+    STC_ENOROM = -4,   /* rom_id not set on this handle                   */
+    STC_ESINGULAR = 1, /* at least one frequency had info > 0             */
+    STC_ENOCONV = 2    /* at least one frequency had info == -1           */
+};
+
+/* factorisation modes */
+enum {
+    STC_PLAIN_FP64 = 0,   /* complex128 LU with partial pivoting (zgetrf)      */
+    STC_REFINED_FP32 = 1  /* complex64 LU + FP64 iterative refinement        */
+};
+
+typedef struct {
+    double seconds;            /* device time of the solve, hipEvent clock     */
+    long long points;          /* (system, frequency) points solved            */
+    int max_refine_steps;      /* largest refinement step count over points    */
+    int n_singular;            /* points with info > 0                         */
+    int n_not_converged;       /* points with info == -1                       */
+    double max_backward_error; /* max over points and columns (if computed)    */
+} stc_stats;
+
+/* ---- handle --------------------------------------------------------------- */
+
+/* Create a handle on HIP device `device` (ordinal). */
+int stc_create(stc_handle* h, int device);
+int stc_destroy(stc_handle h);
+const char* stc_strerror(int code);
+/* 1 when the library was built for and runs on gfx950, else 0 (no GPU). */
+int stc_device_ok(stc_handle h);
+
+/* ---- ROM operators ------------------------------------------------------------
+ * Copies K, M, D (n x n, leading dimension ld, host memory) to the device.
+ * D may be NULL (undamped).  eta is the structural loss factor applied to K;
+ * eta = 0 reproduces the reference, whose K_R already carries (1 + i eta)
+ * (KrylovROMSubstructure.cpp:332-333, :996-997).  Replaces the members
+ * myKComplexReduced / myMComplexReduced / myDComplexReduced
+ * (src/analysis/KrylovROMSubstructure.h:212-216).  rom_id >= 0; setting an id
+ * again replaces it. */
+int stc_rom_set(stc_handle h, int rom_id, int n, const stc_z64* K,
+                const stc_z64* M, const stc_z64* D, int ld, double eta);
+
+/* Input/output projections for the full back-transformation:
+ * B_R (n x nin, ldb) and C_R (nout x n, ldc), myBReduced / myCReduced
+ * (KrylovROMSubstructure.h:217-220). */
+int stc_rom_set_io(stc_handle h, int rom_id, int nin, const stc_z64* B, int ldb,
+                   int nout, const stc_z64* C, int ldc);
+
+int stc_rom_clear(stc_handle h, int rom_id);
+
+/* ---- the sweep --------------------------------------------------------------
+ * For j in [0, nfreq): X_j = A(2 pi freq_hz[j])^{-1} F.
+ *   F   : n x nrhs (ldf), shared by all frequencies (the reference reuses the
+ *         projected load B_R F for every frequency, KrylovROMSubstructure.cpp:1423)
+ *   X   : nfreq blocks of n x nrhs, block j at X + j * ldx * nrhs (frequency-
+ *         major, as `results` is appended per frequency, :1451)
+ *   info: nfreq ints (see above)
+ *   berr: nullable; nfreq * nrhs normwise backward errors
+ *         ||F - A X||_inf / (||A||_inf ||X||_inf + ||F||_inf) computed in FP64
+ *   mode: STC_PLAIN_FP64 or STC_REFINED_FP32; tol / max_iter apply to the
+ *         refined mode (tol <= 0 selects 1e-12, max_iter <= 0 selects 10)
+ * Host memory in, host memory out; synchronous.
+ * Returns STC_OK, STC_ESINGULAR or STC_ENOCONV (results still written), or an
+ * error < 0. */
+int stc_sweep(stc_handle h, int rom_id, int nfreq, const double* freq_hz,
+              int nrhs, const stc_z64* F, int ldf, stc_z64* X, int ldx,
+              int* info, double* berr, int mode, double tol, int max_iter,
+              stc_stats* stats);
+
+/* Same with every array already in device memory (d_ prefix), enqueued on
+ * `stream` (hipStream_t as void*, NULL = handle stream).  d_berr nullable.
+ * Does not synchronise; stats (nullable) are not filled. */
+int stc_sweep_dev(stc_handle h, int rom_id, int nfreq, const double* d_freq_hz,
+                  int nrhs, const stc_z64* d_F, int ldf, stc_z64* d_X, int ldx,
+                  int* d_info, double* d_berr, int mode, double tol,
+                  int max_iter, void* stream);
+
+/* ---- LU factors (LAPACK zgetrf / cgetrf layout) ---------------------------------
+ * Factor A(2 pi freq_hz[j]) for nfreq frequencies and return the in-place
+ * L\U factors (n x n, column-major, lda = n, block j at LU + j*n*n) and the
+ * 1-based pivots (block j at ipiv + j*n).  mode STC_PLAIN_FP64 writes stc_z64,
+ * STC_REFINED_FP32 writes stc_c32 (the factors the refined path uses).
+ * info as above.  Host memory; synchronous. */
+int stc_getrf(stc_handle h, int rom_id, int nfreq, const double* freq_hz,
+              int mode, void* LU, int* ipiv, int* info);
+
+/* ---- uneven batch: many ROMs and frequency sets in one persistent launch ----- */
+typedef struct {
+    int rom_id;
+    int nfreq;
+    const double* freq_hz; /* nfreq                                    */
+    int nrhs;
+    const stc_z64* F;      /* n x nrhs, ldf                             */
+    int ldf;
+    stc_z64* X;            /* nfreq blocks of n x nrhs (ldx)            */
+    int ldx;
+    int* info;             /* nfreq                                     */
+    double* berr;          /* nullable, nfreq * nrhs                    */
+} stc_sys_desc;
+
+/* Host-memory batch: copies in, one persistent launch over every
+ * (system, frequency) point, copies out.  Synchronous. */
+int stc_sweep_batch(stc_handle h, int nsys, const stc_sys_desc* sys, int mode,
+                    double tol, int max_iter, stc_stats* stats);
+
+/* A plan holds a device-side work queue for a batch whose stc_sys_desc arrays
+ * are DEVICE pointers, ordered longest-first, and a hipGraph of the launch.
+ * stc_plan_execute replays it on `stream` (NULL = handle stream). */
+int stc_plan_create(stc_handle h, int nsys, const stc_sys_desc* dev_sys,
+                    int mode, double tol, int max_iter, stc_plan* plan);
+int stc_plan_execute(stc_plan plan, void* stream);
+int stc_plan_destroy(stc_plan plan);
+
+/* ---- adjoint -------------------------------------------------------------------
+ * Objective J = sum_j sum_r |c^H X_r(omega_j)|^2 (c: n complex).  Parameters:
+ * per-mode stiffness scales s_k with dA/ds_k = (1 + i eta) kdiag[k] phi_k phi_k^T
+ * (Phi: n x p real, ldphi; D held fixed) and the loss factor, dA/deta = i K.
+ * Per frequency the kernel factors A, solves A X = F, then A^H mu = c (c^H X)
+ * with the same factors and accumulates dJ/dtheta = -2 Re(mu^H dA/dtheta X).
+ * mode as for stc_sweep (refined: both solves are refined to tol).
+ * No counterpart in the reference. */
+int stc_adjoint(stc_handle h, int rom_id, int nfreq, const double* freq_hz,
+                int nrhs, const stc_z64* F, int ldf, const stc_z64* c, int p,
+                const double* Phi, int ldphi, const double* kdiag, int mode,
+                double tol, int max_iter, double* J, double* dJ_ds,
+                double* dJ_deta, int* info);
+
+/* ---- full back-transformation (KrylovROMSubstructure.cpp:1393-1460) ----------
+ * For every frequency: X = A^{-1} (B_R inputLoad), Y_j = C_R X (nout x nrhs,
+ * block j at Y + j*nout*nrhs, the order of `results`, :1451) and, when H is
+ * not NULL, H_j = C_R A^{-1} B_R (nout x nin, column-major, block j at
+ * H + j*nout*nin; the reference writes it as one .mtx file per frequency,
+ * :1433-1445).  Requires stc_rom_set_io.  Host memory; synchronous. */
+int stc_back_transform(stc_handle h, int rom_id, int nfreq, const double* freq_hz,
+                       const stc_z64* input_load, int nrhs, stc_z64* Y,
+                       stc_z64* H, int* info, int mode, double tol,
+                       int max_iter, stc_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STACCATO_SWEEP_H */

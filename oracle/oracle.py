@@ -1,0 +1,203 @@
+"""ctypes wrapper of the CPU oracle (oracle/libstc_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the staccato_amd package.  The C source
+(stc_oracle.c) restates the reference's per-frequency loop
+(src/analysis/KrylovROMSubstructure.cpp:1403-1431) with LAPACK zgetf2/zgetrs
+semantics.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libstc_oracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = ctypes.CDLL(LIB)
+        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        sig = {
+            "orc_assemble": (None, [i, vp, vp, vp, d, d, vp]),
+            "orc_zgetrf": (i, [i, vp, i, vp]),
+            "orc_zgetrs": (None, [ctypes.c_char, i, i, vp, i, vp, vp, i]),
+            "orc_cgetrf": (i, [i, vp, i, vp]),
+            "orc_cgetrs": (None, [ctypes.c_char, i, i, vp, i, vp, vp, i]),
+            "orc_backward_error": (d, [i, vp, i, vp, vp]),
+            "orc_sweep": (d, [i, vp, vp, vp, d, i, vp, i, vp, i, vp, i, vp, vp, i]),
+            "orc_sweep_refined": (d, [i, vp, vp, vp, d, i, vp, i, vp, i, vp, i, vp, vp, vp,
+                                      d, i, i]),
+            "orc_adjoint": (i, [i, vp, vp, vp, d, i, vp, i, vp, i, vp, i, vp, i, vp, vp, vp,
+                                vp, i]),
+            "orc_sweep_lapack": (d, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                     ctypes.c_char_p, i, vp, vp, vp, d, i, vp, i, vp, i, vp,
+                                     i, vp, i]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _f(a, dtype=np.complex128):
+    a = np.asarray(a, dtype=dtype)
+    if a.ndim == 1:
+        a = a[:, None]
+    return np.asfortranarray(a)
+
+
+def zgetrf(A):
+    """LU with LAPACK zgetf2 semantics; returns (LU, ipiv 1-based, info)."""
+    A = _f(A).copy(order="F")
+    n = A.shape[0]
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = load().orc_zgetrf(n, _p(A), n, _p(ipiv))
+    return A, ipiv, info
+
+
+def cgetrf(A):
+    A = _f(A, np.complex64).copy(order="F")
+    n = A.shape[0]
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = load().orc_cgetrf(n, _p(A), n, _p(ipiv))
+    return A, ipiv, info
+
+
+def zgetrs(LU, ipiv, B, trans="N"):
+    LU = _f(LU)
+    B = _f(B).copy(order="F")
+    n, nrhs = B.shape
+    load().orc_zgetrs(trans.encode(), n, nrhs, _p(LU), n, _p(np.ascontiguousarray(ipiv, np.int32)),
+                      _p(B), n)
+    return B
+
+
+def cgetrs(LU, ipiv, B, trans="N"):
+    LU = _f(LU, np.complex64)
+    B = _f(B, np.complex64).copy(order="F")
+    n, nrhs = B.shape
+    load().orc_cgetrs(trans.encode(), n, nrhs, _p(LU), n, _p(np.ascontiguousarray(ipiv, np.int32)),
+                      _p(B), n)
+    return B
+
+
+def assemble(K, M, D, eta, omega):
+    K, M = _f(K), _f(M)
+    Dd = _f(D) if D is not None else None
+    n = K.shape[0]
+    A = np.zeros((n, n), dtype=np.complex128, order="F")
+    load().orc_assemble(n, _p(K), _p(M), _p(Dd), float(eta), float(omega), _p(A))
+    return A
+
+
+def backward_error(A, x, f):
+    A = _f(A)
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    f = np.ascontiguousarray(f, dtype=np.complex128)
+    return load().orc_backward_error(A.shape[0], _p(A), A.shape[0], _p(x), _p(f))
+
+
+def sweep(K, M, D, eta, freq, F, nthreads=1, berr=True):
+    """The reference loop: X (nfreq, n, nrhs), info, berr (nfreq, nrhs), seconds."""
+    K, M = _f(K), _f(M)
+    Dd = _f(D) if D is not None else None
+    F = _f(F)
+    n, nrhs = F.shape
+    freq = np.ascontiguousarray(freq, dtype=np.float64)
+    nf = freq.size
+    X = np.zeros((nf, nrhs, n), dtype=np.complex128)
+    info = np.zeros(nf, dtype=np.int32)
+    be = np.zeros((nf, nrhs)) if berr else None
+    t = load().orc_sweep(n, _p(K), _p(M), _p(Dd), float(eta), nf, _p(freq), nrhs, _p(F), n,
+                         _p(X), n, _p(info), _p(be), int(nthreads))
+    return X.transpose(0, 2, 1), info, be, t
+
+
+def sweep_refined(K, M, D, eta, freq, F, tol=1e-10, max_iter=10, nthreads=1):
+    K, M = _f(K), _f(M)
+    Dd = _f(D) if D is not None else None
+    F = _f(F)
+    n, nrhs = F.shape
+    freq = np.ascontiguousarray(freq, dtype=np.float64)
+    nf = freq.size
+    X = np.zeros((nf, nrhs, n), dtype=np.complex128)
+    info = np.zeros(nf, dtype=np.int32)
+    iters = np.zeros(nf, dtype=np.int32)
+    be = np.zeros((nf, nrhs))
+    t = load().orc_sweep_refined(n, _p(K), _p(M), _p(Dd), float(eta), nf, _p(freq), nrhs, _p(F),
+                                 n, _p(X), n, _p(info), _p(iters), _p(be), float(tol),
+                                 int(max_iter), int(nthreads))
+    return X.transpose(0, 2, 1), info, iters, be, t
+
+
+def adjoint(K, M, D, eta, freq, F, c, Phi, kdiag, nthreads=1):
+    K, M = _f(K), _f(M)
+    Dd = _f(D) if D is not None else None
+    F = _f(F)
+    n, nrhs = F.shape
+    freq = np.ascontiguousarray(freq, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.complex128)
+    Phi = np.asfortranarray(Phi, dtype=np.float64)
+    kdiag = np.ascontiguousarray(kdiag, dtype=np.float64)
+    p = Phi.shape[1]
+    J, de = ctypes.c_double(), ctypes.c_double()
+    ds = np.zeros(p)
+    ns = load().orc_adjoint(n, _p(K), _p(M), _p(Dd), float(eta), freq.size, _p(freq), nrhs,
+                            _p(F), n, _p(c), p, _p(Phi), n, _p(kdiag), ctypes.byref(J), _p(ds),
+                            ctypes.byref(de), int(nthreads))
+    return J.value, ds, de.value, ns
+
+
+def openblas_path():
+    """The LAPACK the CPU baseline links at run time: scipy's bundled OpenBLAS."""
+    try:
+        import scipy
+    except ImportError:
+        return None
+    d = os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs")
+    if not os.path.isdir(d):
+        return None
+    for f in sorted(os.listdir(d)):
+        if f.startswith("libscipy_openblas") and f.endswith(".so"):
+            return os.path.join(d, f)
+    return None
+
+
+def sweep_lapack(K, M, D, eta, freq, F, nthreads=1):
+    """Reference loop with OpenBLAS zgetrf_/zgetrs_; returns (X, info, seconds) or None."""
+    lib = openblas_path()
+    if lib is None:
+        return None
+    K, M = _f(K), _f(M)
+    Dd = _f(D) if D is not None else None
+    F = _f(F)
+    n, nrhs = F.shape
+    freq = np.ascontiguousarray(freq, dtype=np.float64)
+    nf = freq.size
+    X = np.zeros((nf, nrhs, n), dtype=np.complex128)
+    info = np.zeros(nf, dtype=np.int32)
+    t = load().orc_sweep_lapack(lib.encode(), b"scipy_zgetrf_", b"scipy_zgetrs_",
+                                b"scipy_openblas_set_num_threads", n, _p(K), _p(M), _p(Dd),
+                                float(eta), nf, _p(freq), nrhs, _p(F), n, _p(X), n, _p(info),
+                                int(nthreads))
+    if t < 0:
+        return None
+    return X.transpose(0, 2, 1), info, t

@@ -1,0 +1,930 @@
+// stc_api.hip -- C ABI of libstaccato_sweep.so (declared in include/staccato_sweep.h).
+//
+// Host side of the MI355X frequency sweep: a handle owns one device, one HIP
+// stream, the ROM operators in device memory and the per-workgroup factor
+// slots.  Every sweep is one persistent launch of sweep_kernel over a device
+// work queue of (system, frequency) points, preceded by a one-lane reset of
+// the queue head (a kernel, so the pair can be captured in a hipGraph).
+#include "../../include/staccato_sweep.h"
+#include "stc_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace stc;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t b) {
+        if (b <= bytes) return STC_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipMalloc(&p, b) != hipSuccess) return STC_ENOMEM;
+        bytes = b;
+        return STC_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename U>
+    U* as() const { return reinterpret_cast<U*>(p); }
+};
+
+struct RomDev {
+    int n = 0;
+    double eta = 0.0;
+    int real_ops = 1;
+    int has_D = 0;
+    DevBuf K, M, D;
+    int nin = 0, nout = 0;
+    DevBuf B, C;  // n x nin, nout x n (complex, ld = n / nout)
+};
+
+}  // namespace
+
+struct stc_ctx {
+    int device = 0;
+    int num_cu = 0;
+    int ok_arch = 0;
+    hipStream_t stream = nullptr;
+    std::map<int, RomDev> roms;
+    DevBuf scratch, counter;
+    // staging for host-memory calls
+    DevBuf freq, F, X, info, berr, iters, fac, ipiv, sys, items, adj, c, phi, kd;
+};
+
+struct stc_plan_s {
+    stc_ctx* h = nullptr;
+    DevBuf sys, items;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+#define HIP_TRY(x)                                       \
+    do {                                                 \
+        if ((x) != hipSuccess) return STC_EDEVICE;       \
+    } while (0)
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int d) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != d) (void)hipSetDevice(d);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
+    int rc = b.ensure(bytes ? bytes : 16);
+    if (rc) return rc;
+    if (bytes) HIP_TRY(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
+    return STC_OK;
+}
+
+constexpr int kBudgetSmall = 80 * 1024;      // two workgroups per CU
+constexpr int kBudgetLarge = 160 * 1024 - 256;  // one workgroup per CU
+
+// Pick the LDS budget of a launch and fill each system's layout.
+int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int* lds_out) {
+    int budget = kBudgetSmall;
+    if (const char* e = getenv("STC_LDS_BUDGET")) budget = std::max(16384, std::min(atoi(e), kBudgetLarge));
+    int jb_cap = 64;
+    if (const char* e = getenv("STC_JB_MAX")) jb_cap = std::max(1, std::min(atoi(e), 64));
+    if (!getenv("STC_LDS_BUDGET")) {
+        // a thin panel starves the trailing update: take the whole CU instead
+        for (const SysDesc& s : sys) {
+            Layout L = plan_layout(s.n, s.nrhs, ts, kBudgetSmall, jb_cap, adj);
+            if (L.lds > kBudgetSmall || (!L.small && L.jb < 16)) budget = kBudgetLarge;
+        }
+    }
+    int lds = 0;
+    for (SysDesc& s : sys) {
+        Layout L = plan_layout(s.n, s.nrhs, ts, budget, jb_cap, adj);
+        if (L.lds > budget) return STC_EINVAL;
+        s.lay_small = L.small;
+        s.lay_jb = L.jb;
+        s.lay_cw = L.cw;
+        s.lay_G = L.G;
+        lds = std::max(lds, L.lds);
+    }
+    *lds_out = lds;
+    return STC_OK;
+}
+
+template <typename T, int R>
+void* kernel_ptr() { return reinterpret_cast<void*>(&sweep_kernel<T, R>); }
+
+template <typename T>
+void* pick_kernel(int maxr) {
+    switch (maxr) {
+        case 1: return kernel_ptr<T, 1>();
+        case 2: return kernel_ptr<T, 2>();
+        case 4: return kernel_ptr<T, 4>();
+        case 8: return kernel_ptr<T, 8>();
+        default: return kernel_ptr<T, 16>();
+    }
+}
+
+template <typename T, int R>
+void launch_typed(dim3 g, int lds, hipStream_t st, const KParams& kp) {
+    hipLaunchKernelGGL((sweep_kernel<T, R>), g, dim3(kThreads), lds, st, kp);
+}
+
+template <typename T>
+void launch_kernel(int maxr, dim3 g, int lds, hipStream_t st, const KParams& kp) {
+    switch (maxr) {
+        case 1: launch_typed<T, 1>(g, lds, st, kp); break;
+        case 2: launch_typed<T, 2>(g, lds, st, kp); break;
+        case 4: launch_typed<T, 4>(g, lds, st, kp); break;
+        case 8: launch_typed<T, 8>(g, lds, st, kp); break;
+        default: launch_typed<T, 16>(g, lds, st, kp); break;
+    }
+}
+
+struct LaunchCfg {
+    int maxr = 1;
+    int lds = 0;
+    int grid = 1;
+    size_t slot_elems = 0;
+};
+
+// Size the launch: grid = resident workgroups (capped by points), scratch slots.
+int configure(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int refine, int adj,
+              int factor_out, LaunchCfg* cfg) {
+    const int ts = refine ? (int)sizeof(cx<float>) : (int)sizeof(cx<double>);
+    int maxn = 0;
+    for (const SysDesc& s : sys) maxn = std::max(maxn, s.n);
+    if (maxn < 1 || maxn > 1024) return STC_EINVAL;
+    int rc = plan_systems(sys, ts, adj, &cfg->lds);
+    if (rc) return rc;
+    int r = (maxn + 63) / 64, maxr = 1;
+    while (maxr < r) maxr <<= 1;
+    cfg->maxr = maxr;
+    void* kfun = refine ? pick_kernel<float>(maxr) : pick_kernel<double>(maxr);
+    HIP_TRY(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, cfg->lds));
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfun, kThreads, cfg->lds));
+    if (per_cu < 1) per_cu = 1;
+    long long g = (long long)h->num_cu * per_cu;
+    if (const char* e = getenv("STC_GRID")) g = std::max(1, atoi(e));
+    cfg->grid = (int)std::max(1LL, std::min(g, nitems));
+    size_t slot = 0;
+    if (!factor_out)
+        for (const SysDesc& s : sys)
+            if (!s.lay_small) slot = std::max(slot, (size_t)s.n * s.n);
+    cfg->slot_elems = slot;
+    if (slot) {
+        rc = h->scratch.ensure(slot * ts * (size_t)cfg->grid);
+        if (rc) return rc;
+    }
+    return STC_OK;
+}
+
+int enqueue(stc_ctx* h, const LaunchCfg& cfg, KParams& kp, int refine, hipStream_t st) {
+    kp.counter = h->counter.as<int>();
+    kp.scratch = h->scratch.p;
+    kp.slot_elems = cfg.slot_elems;
+    kp.lds_budget = cfg.lds;
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, st, kp.counter);
+    if (refine) launch_kernel<float>(cfg.maxr, dim3(cfg.grid), cfg.lds, st, kp);
+    else launch_kernel<double>(cfg.maxr, dim3(cfg.grid), cfg.lds, st, kp);
+    HIP_TRY(hipGetLastError());
+    return STC_OK;
+}
+
+SysDesc base_desc(const RomDev& r) {
+    SysDesc s;
+    memset(&s, 0, sizeof(s));
+    s.n = r.n;
+    s.eta = r.eta;
+    s.real_ops = r.real_ops;
+    s.has_D = r.has_D;
+    s.K = r.K.as<double>();
+    s.M = r.M.as<double>();
+    s.D = r.has_D ? r.D.as<double>() : nullptr;
+    return s;
+}
+
+KParams base_params(int mode, double tol, int max_iter) {
+    KParams kp;
+    memset(&kp, 0, sizeof(kp));
+    kp.flags = (mode == STC_REFINED_FP32) ? kRefine : 0;
+    kp.tol = tol > 0.0 ? tol : 1e-12;
+    kp.max_iter = max_iter > 0 ? max_iter : 10;
+    return kp;
+}
+
+int check_mode(int mode) { return (mode == STC_PLAIN_FP64 || mode == STC_REFINED_FP32) ? 0 : 1; }
+
+int status_from_info(const int* info, long long cnt, stc_stats* st) {
+    int nsing = 0, nconv = 0;
+    for (long long j = 0; j < cnt; ++j) {
+        if (info[j] > 0) ++nsing;
+        else if (info[j] < 0) ++nconv;
+    }
+    if (st) {
+        st->n_singular = nsing;
+        st->n_not_converged = nconv;
+    }
+    if (nsing) return STC_ESINGULAR;
+    if (nconv) return STC_ENOCONV;
+    return STC_OK;
+}
+
+// Copy a host n x ncol block with leading dimension ld into a packed device buffer.
+int upload_block(DevBuf& b, const stc_z64* src, int n, int ncol, int ld, hipStream_t st) {
+    std::vector<stc_z64> tmp((size_t)n * ncol);
+    for (int c = 0; c < ncol; ++c)
+        memcpy(&tmp[(size_t)c * n], src + (size_t)c * ld, sizeof(stc_z64) * n);
+    int rc = upload(b, tmp.data(), tmp.size() * sizeof(stc_z64), st);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    return STC_OK;
+}
+
+// Y(o, j) = sum_i C(o, i) X(i, j) for every frequency block (batched projection).
+__global__ void project_kernel(const double2* C, int nout, int n, const double2* X, int ncol,
+                               double2* Y) {
+    const int f = blockIdx.x;
+    const double2* Xf = X + (size_t)f * n * ncol;
+    double2* Yf = Y + (size_t)f * nout * ncol;
+    for (int e = threadIdx.x; e < nout * ncol; e += blockDim.x) {
+        const int o = e % nout, j = e / nout;
+        double re = 0.0, im = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double2 c = C[o + (size_t)i * nout];
+            const double2 x = Xf[i + (size_t)j * n];
+            re += c.x * x.x - c.y * x.y;
+            im += c.x * x.y + c.y * x.x;
+        }
+        Yf[e] = make_double2(re, im);
+    }
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char* stc_strerror(int code) {
+    switch (code) {
+        case STC_OK: return "ok";
+        case STC_EINVAL: return "invalid argument";
+        case STC_ENOMEM: return "device allocation failed";
+        case STC_EDEVICE: return "HIP runtime error";
+        case STC_ENOROM: return "rom_id not set";
+        case STC_ESINGULAR: return "singular A(omega) at one or more frequencies";
+        case STC_ENOCONV: return "iterative refinement did not reach tol at one or more frequencies";
+        default: return "unknown status";
+    }
+}
+
+int stc_create(stc_handle* out, int device) {
+    if (!out) return STC_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return STC_EDEVICE;
+    stc_ctx* h = new stc_ctx();
+    h->device = device;
+    DeviceGuard g(device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete h; return STC_EDEVICE; }
+    h->num_cu = prop.multiProcessorCount;
+    h->ok_arch = strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return STC_EDEVICE;
+    }
+    if (h->counter.ensure(256) != STC_OK) {
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return STC_ENOMEM;
+    }
+    *out = h;
+    return STC_OK;
+}
+
+int stc_destroy(stc_handle h) {
+    if (!h) return STC_EINVAL;
+    {
+        DeviceGuard g(h->device);
+        (void)hipStreamSynchronize(h->stream);
+        for (auto& kv : h->roms) {
+            kv.second.K.release(); kv.second.M.release(); kv.second.D.release();
+            kv.second.B.release(); kv.second.C.release();
+        }
+        DevBuf* bufs[] = {&h->scratch, &h->counter, &h->freq, &h->F, &h->X, &h->info, &h->berr,
+                          &h->iters, &h->fac, &h->ipiv, &h->sys, &h->items, &h->adj, &h->c,
+                          &h->phi, &h->kd};
+        for (DevBuf* b : bufs) b->release();
+        (void)hipStreamDestroy(h->stream);
+    }
+    delete h;
+    return STC_OK;
+}
+
+int stc_device_ok(stc_handle h) { return h ? h->ok_arch : 0; }
+
+int stc_rom_set(stc_handle h, int rom_id, int n, const stc_z64* K, const stc_z64* M,
+                const stc_z64* D, int ld, double eta) {
+    if (!h || rom_id < 0 || n < 1 || n > 1024 || ld < n || !K || !M || !std::isfinite(eta))
+        return STC_EINVAL;
+    DeviceGuard g(h->device);
+    RomDev& r = h->roms[rom_id];
+    r.n = n;
+    r.eta = eta;
+    r.has_D = D != nullptr;
+    bool real = true;
+    const stc_z64* ops[3] = {K, M, D};
+    for (int o = 0; o < 3 && real; ++o) {
+        if (!ops[o]) continue;
+        for (int j = 0; j < n && real; ++j)
+            for (int i = 0; i < n; ++i)
+                if (ops[o][(size_t)j * ld + i].im != 0.0) { real = false; break; }
+    }
+    r.real_ops = real;
+    DevBuf* dst[3] = {&r.K, &r.M, &r.D};
+    for (int o = 0; o < 3; ++o) {
+        if (!ops[o]) continue;
+        if (real) {
+            std::vector<double> tmp((size_t)n * n);
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) tmp[(size_t)j * n + i] = ops[o][(size_t)j * ld + i].re;
+            int rc = upload(*dst[o], tmp.data(), tmp.size() * sizeof(double), h->stream);
+            if (rc) return rc;
+            HIP_TRY(hipStreamSynchronize(h->stream));
+        } else {
+            int rc = upload_block(*dst[o], ops[o], n, n, ld, h->stream);
+            if (rc) return rc;
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return STC_OK;
+}
+
+int stc_rom_set_io(stc_handle h, int rom_id, int nin, const stc_z64* B, int ldb, int nout,
+                   const stc_z64* C, int ldc) {
+    if (!h) return STC_EINVAL;
+    auto it = h->roms.find(rom_id);
+    if (it == h->roms.end()) return STC_ENOROM;
+    RomDev& r = it->second;
+    if (nin < 1 || nout < 1 || !B || !C || ldb < r.n || ldc < nout) return STC_EINVAL;
+    DeviceGuard g(h->device);
+    int rc = upload_block(r.B, B, r.n, nin, ldb, h->stream);
+    if (rc) return rc;
+    rc = upload_block(r.C, C, nout, r.n, ldc, h->stream);
+    if (rc) return rc;
+    r.nin = nin;
+    r.nout = nout;
+    return STC_OK;
+}
+
+int stc_rom_clear(stc_handle h, int rom_id) {
+    if (!h) return STC_EINVAL;
+    auto it = h->roms.find(rom_id);
+    if (it == h->roms.end()) return STC_ENOROM;
+    DeviceGuard g(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    it->second.K.release(); it->second.M.release(); it->second.D.release();
+    it->second.B.release(); it->second.C.release();
+    h->roms.erase(it);
+    return STC_OK;
+}
+
+int stc_sweep_dev(stc_handle h, int rom_id, int nfreq, const double* d_freq_hz, int nrhs,
+                  const stc_z64* d_F, int ldf, stc_z64* d_X, int ldx, int* d_info,
+                  double* d_berr, int mode, double tol, int max_iter, void* stream) {
+    if (!h || nfreq < 0 || nrhs < 1 || check_mode(mode)) return STC_EINVAL;
+    auto it = h->roms.find(rom_id);
+    if (it == h->roms.end()) return STC_ENOROM;
+    const RomDev& r = it->second;
+    if (ldf < r.n || ldx < r.n || (nfreq && (!d_freq_hz || !d_F || !d_X || !d_info)))
+        return STC_EINVAL;
+    if (nfreq == 0) return STC_OK;
+    DeviceGuard g(h->device);
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
+    KParams kp = base_params(mode, tol, max_iter);
+    std::vector<SysDesc> sys(1, base_desc(r));
+    SysDesc& s = sys[0];
+    s.nrhs = nrhs;
+    s.nfreq = nfreq;
+    s.ldf = ldf;
+    s.ldx = ldx;
+    s.freq = d_freq_hz;
+    s.F = reinterpret_cast<const double2*>(d_F);
+    s.X = reinterpret_cast<double2*>(d_X);
+    s.info = d_info;
+    s.berr = d_berr;
+    LaunchCfg cfg;
+    int rc = configure(h, sys, nfreq, mode == STC_REFINED_FP32, 0, 0, &cfg);
+    if (rc) return rc;
+    kp.one = sys[0];
+    kp.nitems = nfreq;
+    if (d_berr) kp.flags |= kWantBerr;
+    return enqueue(h, cfg, kp, mode == STC_REFINED_FP32, st);
+}
+
+int stc_sweep(stc_handle h, int rom_id, int nfreq, const double* freq_hz, int nrhs,
+              const stc_z64* F, int ldf, stc_z64* X, int ldx, int* info, double* berr, int mode,
+              double tol, int max_iter, stc_stats* stats) {
+    if (!h || nfreq < 0 || nrhs < 1 || check_mode(mode)) return STC_EINVAL;
+    auto it = h->roms.find(rom_id);
+    if (it == h->roms.end()) return STC_ENOROM;
+    const int n = it->second.n;
+    if (ldf < n || ldx < n || (nfreq && (!freq_hz || !F || !X || !info))) return STC_EINVAL;
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (nfreq == 0) return STC_OK;
+    DeviceGuard g(h->device);
+    hipStream_t st = h->stream;
+    int rc = upload(h->freq, freq_hz, sizeof(double) * nfreq, st);
+    if (rc) return rc;
+    rc = upload_block(h->F, F, n, nrhs, ldf, st);
+    if (rc) return rc;
+    const size_t xcount = (size_t)nfreq * n * nrhs;
+    if ((rc = h->X.ensure(xcount * sizeof(stc_z64)))) return rc;
+    if ((rc = h->info.ensure(sizeof(int) * nfreq))) return rc;
+    if ((rc = h->iters.ensure(sizeof(int) * nfreq))) return rc;
+    if (berr && (rc = h->berr.ensure(sizeof(double) * nfreq * nrhs))) return rc;
+
+    const auto& r = it->second;
+    KParams kp = base_params(mode, tol, max_iter);
+    std::vector<SysDesc> sys(1, base_desc(r));
+    SysDesc& s = sys[0];
+    s.nrhs = nrhs; s.nfreq = nfreq; s.ldf = n; s.ldx = n;
+    s.freq = h->freq.as<double>();
+    s.F = h->F.as<double2>();
+    s.X = h->X.as<double2>();
+    s.info = h->info.as<int>();
+    s.berr = berr ? h->berr.as<double>() : nullptr;
+    s.iters = h->iters.as<int>();
+    LaunchCfg cfg;
+    rc = configure(h, sys, nfreq, mode == STC_REFINED_FP32, 0, 0, &cfg);
+    if (rc) return rc;
+    kp.one = sys[0];
+    kp.nitems = nfreq;
+    if (berr) kp.flags |= kWantBerr;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+    rc = enqueue(h, cfg, kp, mode == STC_REFINED_FP32, st);
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    // copy out (X packed n x nrhs per frequency -> caller's ldx)
+    std::vector<stc_z64> xh(xcount);
+    HIP_TRY(hipMemcpy(xh.data(), h->X.p, xcount * sizeof(stc_z64), hipMemcpyDeviceToHost));
+    for (int f = 0; f < nfreq; ++f)
+        for (int c = 0; c < nrhs; ++c)
+            memcpy(X + ((size_t)f * nrhs + c) * ldx, &xh[((size_t)f * nrhs + c) * n],
+                   sizeof(stc_z64) * n);
+    HIP_TRY(hipMemcpy(info, h->info.p, sizeof(int) * nfreq, hipMemcpyDeviceToHost));
+    std::vector<int> its(nfreq);
+    HIP_TRY(hipMemcpy(its.data(), h->iters.p, sizeof(int) * nfreq, hipMemcpyDeviceToHost));
+    if (berr)
+        HIP_TRY(hipMemcpy(berr, h->berr.p, sizeof(double) * nfreq * nrhs, hipMemcpyDeviceToHost));
+    if (stats) {
+        stats->seconds = ms * 1e-3;
+        stats->points = nfreq;
+        stats->max_refine_steps = *std::max_element(its.begin(), its.end());
+        if (berr) {
+            double m = 0.0;
+            for (long long q = 0; q < (long long)nfreq * nrhs; ++q)
+                if (std::isfinite(berr[q])) m = std::max(m, berr[q]);
+            stats->max_backward_error = m;
+        }
+    }
+    return status_from_info(info, nfreq, stats);
+}
+
+int stc_getrf(stc_handle h, int rom_id, int nfreq, const double* freq_hz, int mode, void* LU,
+              int* ipiv, int* info) {
+    if (!h || nfreq < 0 || check_mode(mode)) return STC_EINVAL;
+    auto it = h->roms.find(rom_id);
+    if (it == h->roms.end()) return STC_ENOROM;
+    if (nfreq == 0) return STC_OK;
+    if (!freq_hz || !LU || !ipiv || !info) return STC_EINVAL;
+    const int n = it->second.n;
+    const bool refine = mode == STC_REFINED_FP32;
+    const size_t ts = refine ? sizeof(stc_c32) : sizeof(stc_z64);
+    DeviceGuard g(h->device);
+    hipStream_t st = h->stream;
+    int rc = upload(h->freq, freq_hz, sizeof(double) * nfreq, st);
+    if (rc) return rc;
+    if ((rc = h->fac.ensure(ts * (size_t)n * n * nfreq))) return rc;
+    if ((rc = h->ipiv.ensure(sizeof(int) * (size_t)n * nfreq))) return rc;
+    if ((rc = h->info.ensure(sizeof(int) * nfreq))) return rc;
+    KParams kp = base_params(mode, 0.0, 0);
+    kp.flags |= kFactorOut;
+    std::vector<SysDesc> sys(1, base_desc(it->second));
+    SysDesc& s = sys[0];
+    s.nrhs = 1; s.nfreq = nfreq; s.ldf = n; s.ldx = n;
+    s.freq = h->freq.as<double>();
+    s.info = h->info.as<int>();
+    s.factor_out = h->fac.p;
+    s.ipiv_out = h->ipiv.as<int>();
+    LaunchCfg cfg;
+    rc = configure(h, sys, nfreq, refine, 0, 1, &cfg);
+    if (rc) return rc;
+    kp.one = sys[0];
+    kp.nitems = nfreq;
+    rc = enqueue(h, cfg, kp, refine, st);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipMemcpy(LU, h->fac.p, ts * (size_t)n * n * nfreq, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ipiv, h->ipiv.p, sizeof(int) * (size_t)n * nfreq, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(info, h->info.p, sizeof(int) * nfreq, hipMemcpyDeviceToHost));
+    return status_from_info(info, nfreq, nullptr);
+}
+
+// ---- uneven batch ---------------------------------------------------------------
+namespace {
+
+// Build descriptors + a longest-first (n^3) item table.
+int build_batch(stc_ctx* h, int nsys, const stc_sys_desc* in, std::vector<SysDesc>& sys,
+                std::vector<int2>& items) {
+    sys.clear();
+    items.clear();
+    for (int i = 0; i < nsys; ++i) {
+        auto it = h->roms.find(in[i].rom_id);
+        if (it == h->roms.end()) return STC_ENOROM;
+        const RomDev& r = it->second;
+        if (in[i].nfreq < 0 || in[i].nrhs < 1 || in[i].ldf < r.n || in[i].ldx < r.n)
+            return STC_EINVAL;
+        SysDesc s = base_desc(r);
+        s.nrhs = in[i].nrhs; s.nfreq = in[i].nfreq; s.ldf = in[i].ldf; s.ldx = in[i].ldx;
+        s.freq = in[i].freq_hz;
+        s.F = reinterpret_cast<const double2*>(in[i].F);
+        s.X = reinterpret_cast<double2*>(in[i].X);
+        s.info = in[i].info;
+        s.berr = in[i].berr;
+        sys.push_back(s);
+    }
+    std::vector<int> order(nsys);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const double ca = (double)sys[a].n * sys[a].n * (sys[a].n + 3.0 * sys[a].nrhs);
+        const double cb = (double)sys[b].n * sys[b].n * (sys[b].n + 3.0 * sys[b].nrhs);
+        return ca > cb;
+    });
+    for (int i : order)
+        for (int f = 0; f < sys[i].nfreq; ++f) items.push_back(make_int2(i, f));
+    return STC_OK;
+}
+
+}  // namespace
+
+int stc_sweep_batch(stc_handle h, int nsys, const stc_sys_desc* sysd, int mode, double tol,
+                    int max_iter, stc_stats* stats) {
+    if (!h || nsys < 0 || (nsys && !sysd) || check_mode(mode)) return STC_EINVAL;
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (nsys == 0) return STC_OK;
+    DeviceGuard g(h->device);
+    hipStream_t st = h->stream;
+    // stage every system's host arrays in one device buffer each
+    size_t nf = 0, nF = 0, nX = 0, nB = 0;
+    for (int i = 0; i < nsys; ++i) {
+        auto it = h->roms.find(sysd[i].rom_id);
+        if (it == h->roms.end()) return STC_ENOROM;
+        const int n = it->second.n;
+        if (sysd[i].nfreq < 0 || sysd[i].nrhs < 1 || (sysd[i].nfreq && (!sysd[i].freq_hz ||
+            !sysd[i].F || !sysd[i].X || !sysd[i].info)) || sysd[i].ldf < n || sysd[i].ldx < n)
+            return STC_EINVAL;
+        nf += sysd[i].nfreq;
+        nF += (size_t)n * sysd[i].nrhs;
+        nX += (size_t)n * sysd[i].nrhs * sysd[i].nfreq;
+        nB += (size_t)sysd[i].nrhs * sysd[i].nfreq;
+    }
+    std::vector<double> hf;
+    std::vector<stc_z64> hF;
+    hf.reserve(nf);
+    hF.reserve(nF);
+    std::vector<stc_sys_desc> dd(sysd, sysd + nsys);
+    std::vector<size_t> ofs_f(nsys), ofs_F(nsys), ofs_X(nsys), ofs_B(nsys);
+    size_t cx_ = 0, cb_ = 0;
+    for (int i = 0; i < nsys; ++i) {
+        const int n = h->roms[sysd[i].rom_id].n;
+        ofs_f[i] = hf.size();
+        hf.insert(hf.end(), sysd[i].freq_hz, sysd[i].freq_hz + sysd[i].nfreq);
+        ofs_F[i] = hF.size();
+        for (int c = 0; c < sysd[i].nrhs; ++c)
+            hF.insert(hF.end(), sysd[i].F + (size_t)c * sysd[i].ldf,
+                      sysd[i].F + (size_t)c * sysd[i].ldf + n);
+        ofs_X[i] = cx_;
+        cx_ += (size_t)n * sysd[i].nrhs * sysd[i].nfreq;
+        ofs_B[i] = cb_;
+        cb_ += (size_t)sysd[i].nrhs * sysd[i].nfreq;
+    }
+    int rc;
+    if ((rc = upload(h->freq, hf.data(), hf.size() * sizeof(double), st))) return rc;
+    if ((rc = upload(h->F, hF.data(), hF.size() * sizeof(stc_z64), st))) return rc;
+    if ((rc = h->X.ensure(std::max<size_t>(1, nX) * sizeof(stc_z64)))) return rc;
+    if ((rc = h->info.ensure(std::max<size_t>(1, nf) * sizeof(int)))) return rc;
+    if ((rc = h->berr.ensure(std::max<size_t>(1, nB) * sizeof(double)))) return rc;
+    for (int i = 0; i < nsys; ++i) {
+        const int n = h->roms[sysd[i].rom_id].n;
+        dd[i].freq_hz = h->freq.as<double>() + ofs_f[i];
+        dd[i].F = reinterpret_cast<const stc_z64*>(h->F.as<stc_z64>() + ofs_F[i]);
+        dd[i].ldf = n;
+        dd[i].X = h->X.as<stc_z64>() + ofs_X[i];
+        dd[i].ldx = n;
+        dd[i].info = h->info.as<int>() + ofs_f[i];
+        dd[i].berr = sysd[i].berr ? h->berr.as<double>() + ofs_B[i] : nullptr;
+    }
+    std::vector<SysDesc> sys;
+    std::vector<int2> items;
+    if ((rc = build_batch(h, nsys, dd.data(), sys, items))) return rc;
+    if (items.empty()) return STC_OK;
+    const bool refine = mode == STC_REFINED_FP32;
+    LaunchCfg cfg;
+    if ((rc = configure(h, sys, (long long)items.size(), refine, 0, 0, &cfg))) return rc;
+    if ((rc = upload(h->sys, sys.data(), sys.size() * sizeof(SysDesc), st))) return rc;
+    if ((rc = upload(h->items, items.data(), items.size() * sizeof(int2), st))) return rc;
+    KParams kp = base_params(mode, tol, max_iter);
+    kp.sys = h->sys.as<SysDesc>();
+    kp.nsys = nsys;
+    kp.items = h->items.as<int2>();
+    kp.nitems = (int)items.size();
+    kp.flags |= kWantBerr;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+    rc = enqueue(h, cfg, kp, refine, st);
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    std::vector<stc_z64> xh(std::max<size_t>(1, nX));
+    std::vector<int> ih(std::max<size_t>(1, nf));
+    std::vector<double> bh(std::max<size_t>(1, nB));
+    HIP_TRY(hipMemcpy(xh.data(), h->X.p, nX * sizeof(stc_z64), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ih.data(), h->info.p, nf * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bh.data(), h->berr.p, nB * sizeof(double), hipMemcpyDeviceToHost));
+    double bmax = 0.0;
+    for (int i = 0; i < nsys; ++i) {
+        const int n = h->roms[sysd[i].rom_id].n;
+        for (int f = 0; f < sysd[i].nfreq; ++f)
+            for (int c = 0; c < sysd[i].nrhs; ++c)
+                memcpy(sysd[i].X + ((size_t)f * sysd[i].nrhs + c) * sysd[i].ldx,
+                       &xh[ofs_X[i] + ((size_t)f * sysd[i].nrhs + c) * n], sizeof(stc_z64) * n);
+        memcpy(sysd[i].info, &ih[ofs_f[i]], sizeof(int) * sysd[i].nfreq);
+        if (sysd[i].berr) {
+            memcpy(sysd[i].berr, &bh[ofs_B[i]], sizeof(double) * sysd[i].nfreq * sysd[i].nrhs);
+            for (size_t q = 0; q < (size_t)sysd[i].nfreq * sysd[i].nrhs; ++q)
+                if (std::isfinite(sysd[i].berr[q])) bmax = std::max(bmax, sysd[i].berr[q]);
+        }
+    }
+    if (stats) {
+        stats->seconds = ms * 1e-3;
+        stats->points = (long long)items.size();
+        stats->max_backward_error = bmax;
+    }
+    return status_from_info(ih.data(), (long long)nf, stats);
+}
+
+int stc_plan_create(stc_handle h, int nsys, const stc_sys_desc* dev_sys, int mode, double tol,
+                    int max_iter, stc_plan* out) {
+    if (!h || !out || nsys < 1 || !dev_sys || check_mode(mode)) return STC_EINVAL;
+    *out = nullptr;
+    DeviceGuard g(h->device);
+    std::vector<SysDesc> sys;
+    std::vector<int2> items;
+    int rc = build_batch(h, nsys, dev_sys, sys, items);
+    if (rc) return rc;
+    if (items.empty()) return STC_EINVAL;
+    const bool refine = mode == STC_REFINED_FP32;
+    LaunchCfg cfg;
+    if ((rc = configure(h, sys, (long long)items.size(), refine, 0, 0, &cfg))) return rc;
+    stc_plan_s* p = new stc_plan_s();
+    p->h = h;
+    hipStream_t st = h->stream;
+    if ((rc = upload(p->sys, sys.data(), sys.size() * sizeof(SysDesc), st)) ||
+        (rc = upload(p->items, items.data(), items.size() * sizeof(int2), st))) {
+        delete p;
+        return rc;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) { delete p; return STC_EDEVICE; }
+    KParams kp = base_params(mode, tol, max_iter);
+    kp.sys = p->sys.as<SysDesc>();
+    kp.nsys = nsys;
+    kp.items = p->items.as<int2>();
+    kp.nitems = (int)items.size();
+    for (const SysDesc& s : sys)
+        if (s.berr) kp.flags |= kWantBerr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        delete p;
+        return STC_EDEVICE;
+    }
+    rc = enqueue(h, cfg, kp, refine, st);
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(st, &graph);
+    if (rc || ce != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        p->sys.release(); p->items.release();
+        delete p;
+        return rc ? rc : STC_EDEVICE;
+    }
+    p->graph = graph;
+    if (hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        p->sys.release(); p->items.release();
+        delete p;
+        return STC_EDEVICE;
+    }
+    *out = p;
+    return STC_OK;
+}
+
+int stc_plan_execute(stc_plan p, void* stream) {
+    if (!p || !p->exec) return STC_EINVAL;
+    DeviceGuard g(p->h->device);
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : p->h->stream;
+    HIP_TRY(hipGraphLaunch(p->exec, st));
+    return STC_OK;
+}
+
+int stc_plan_destroy(stc_plan p) {
+    if (!p) return STC_EINVAL;
+    {
+        DeviceGuard g(p->h->device);
+        (void)hipStreamSynchronize(p->h->stream);
+        if (p->exec) (void)hipGraphExecDestroy(p->exec);
+        if (p->graph) (void)hipGraphDestroy(p->graph);
+        p->sys.release();
+        p->items.release();
+    }
+    delete p;
+    return STC_OK;
+}
+
+// ---- adjoint ---------------------------------------------------------------------
+int stc_adjoint(stc_handle h, int rom_id, int nfreq, const double* freq_hz, int nrhs,
+                const stc_z64* F, int ldf, const stc_z64* c, int p, const double* Phi,
+                int ldphi, const double* kdiag, int mode, double tol, int max_iter, double* J,
+                double* dJ_ds, double* dJ_deta, int* info) {
+    if (!h || nfreq < 1 || nrhs < 1 || p < 0 || check_mode(mode) || !freq_hz || !F || !c ||
+        !J || !dJ_deta || !info || (p > 0 && (!Phi || !kdiag || !dJ_ds)))
+        return STC_EINVAL;
+    auto it = h->roms.find(rom_id);
+    if (it == h->roms.end()) return STC_ENOROM;
+    const int n = it->second.n;
+    if (ldf < n || (p > 0 && ldphi < n)) return STC_EINVAL;
+    DeviceGuard g(h->device);
+    hipStream_t st = h->stream;
+    int rc;
+    if ((rc = upload(h->freq, freq_hz, sizeof(double) * nfreq, st))) return rc;
+    if ((rc = upload_block(h->F, F, n, nrhs, ldf, st))) return rc;
+    if ((rc = upload(h->c, c, sizeof(stc_z64) * n, st))) return rc;
+    std::vector<double> ph((size_t)n * std::max(p, 1));
+    for (int k = 0; k < p; ++k) memcpy(&ph[(size_t)k * n], Phi + (size_t)k * ldphi, sizeof(double) * n);
+    if ((rc = upload(h->phi, ph.data(), ph.size() * sizeof(double), st))) return rc;
+    if ((rc = upload(h->kd, p ? kdiag : ph.data(), sizeof(double) * std::max(p, 1), st))) return rc;
+    if ((rc = h->adj.ensure(sizeof(double) * (size_t)nfreq * (p + 2)))) return rc;
+    if ((rc = h->info.ensure(sizeof(int) * nfreq))) return rc;
+    KParams kp = base_params(mode, tol, max_iter);
+    kp.flags |= kAdjoint;
+    std::vector<SysDesc> sys(1, base_desc(it->second));
+    SysDesc& s = sys[0];
+    s.nrhs = nrhs; s.nfreq = nfreq; s.ldf = n; s.ldx = n;
+    s.freq = h->freq.as<double>();
+    s.F = h->F.as<double2>();
+    s.info = h->info.as<int>();
+    s.p = p;
+    s.c = h->c.as<double2>();
+    s.Phi = h->phi.as<double>();
+    s.kdiag = h->kd.as<double>();
+    s.adj_out = h->adj.as<double>();
+    const bool refine = mode == STC_REFINED_FP32;
+    LaunchCfg cfg;
+    if ((rc = configure(h, sys, nfreq, refine, 1, 0, &cfg))) return rc;
+    kp.one = sys[0];
+    kp.nitems = nfreq;
+    if ((rc = enqueue(h, cfg, kp, refine, st))) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<double> ao((size_t)nfreq * (p + 2));
+    HIP_TRY(hipMemcpy(ao.data(), h->adj.p, ao.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(info, h->info.p, sizeof(int) * nfreq, hipMemcpyDeviceToHost));
+    double Js = 0.0, Es = 0.0;
+    std::vector<double> ds(std::max(p, 1), 0.0);
+    for (int f = 0; f < nfreq; ++f) {
+        if (info[f] > 0) continue;
+        const double* o = &ao[(size_t)f * (p + 2)];
+        Js += o[0];
+        Es += o[1];
+        for (int k = 0; k < p; ++k) ds[k] += o[2 + k];
+    }
+    *J = Js;
+    *dJ_deta = Es;
+    for (int k = 0; k < p; ++k) dJ_ds[k] = ds[k];
+    return status_from_info(info, nfreq, nullptr);
+}
+
+// ---- full back-transformation ---------------------------------------------------------
+int stc_back_transform(stc_handle h, int rom_id, int nfreq, const double* freq_hz,
+                       const stc_z64* input_load, int nrhs, stc_z64* Y, stc_z64* H, int* info,
+                       int mode, double tol, int max_iter, stc_stats* stats) {
+    if (!h || nfreq < 0 || nrhs < 1 || check_mode(mode)) return STC_EINVAL;
+    auto it = h->roms.find(rom_id);
+    if (it == h->roms.end()) return STC_ENOROM;
+    const RomDev& r = it->second;
+    if (r.nin < 1 || r.nout < 1) return STC_EINVAL;
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (nfreq == 0) return STC_OK;
+    if (!freq_hz || !input_load || !Y || !info) return STC_EINVAL;
+    const int n = r.n, nin = r.nin, nout = r.nout;
+    const int ncol = nrhs + (H ? nin : 0);
+    DeviceGuard g(h->device);
+    hipStream_t st = h->stream;
+    // right-hand sides [B_R * inputLoad | B_R] (the projection is frequency independent,
+    // hoisted out of the loop of KrylovROMSubstructure.cpp:1423-1424)
+    std::vector<stc_z64> Bh((size_t)n * nin);
+    HIP_TRY(hipMemcpy(Bh.data(), r.B.p, Bh.size() * sizeof(stc_z64), hipMemcpyDeviceToHost));
+    std::vector<stc_z64> rhs((size_t)n * ncol);
+    for (int j = 0; j < nrhs; ++j)
+        for (int i = 0; i < n; ++i) {
+            double re = 0.0, im = 0.0;
+            for (int k = 0; k < nin; ++k) {
+                const stc_z64 b = Bh[(size_t)k * n + i];
+                const stc_z64 l = input_load[(size_t)j * nin + k];
+                re += b.re * l.re - b.im * l.im;
+                im += b.re * l.im + b.im * l.re;
+            }
+            rhs[(size_t)j * n + i] = {re, im};
+        }
+    if (H) memcpy(&rhs[(size_t)nrhs * n], Bh.data(), Bh.size() * sizeof(stc_z64));
+    int rc;
+    if ((rc = upload(h->freq, freq_hz, sizeof(double) * nfreq, st))) return rc;
+    if ((rc = upload(h->F, rhs.data(), rhs.size() * sizeof(stc_z64), st))) return rc;
+    if ((rc = h->X.ensure(sizeof(stc_z64) * (size_t)nfreq * n * ncol))) return rc;
+    if ((rc = h->info.ensure(sizeof(int) * nfreq))) return rc;
+    if ((rc = h->berr.ensure(sizeof(stc_z64) * (size_t)nfreq * nout * ncol))) return rc;
+    KParams kp = base_params(mode, tol, max_iter);
+    std::vector<SysDesc> sys(1, base_desc(r));
+    SysDesc& s = sys[0];
+    s.nrhs = ncol; s.nfreq = nfreq; s.ldf = n; s.ldx = n;
+    s.freq = h->freq.as<double>();
+    s.F = h->F.as<double2>();
+    s.X = h->X.as<double2>();
+    s.info = h->info.as<int>();
+    LaunchCfg cfg;
+    if ((rc = configure(h, sys, nfreq, mode == STC_REFINED_FP32, 0, 0, &cfg))) return rc;
+    kp.one = sys[0];
+    kp.nitems = nfreq;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+    if ((rc = enqueue(h, cfg, kp, mode == STC_REFINED_FP32, st))) return rc;
+    // outputs y = C_R X and H_R = C_R A^{-1} B_R for every frequency (:1438-1448)
+    double2* Yd = h->berr.as<double2>();
+    if ((size_t)nfreq * nout * ncol * sizeof(double2) > h->berr.bytes) return STC_EINVAL;
+    hipLaunchKernelGGL(project_kernel, dim3(nfreq), dim3(256), 0, st, r.C.as<double2>(), nout, n,
+                       h->X.as<double2>(), ncol, Yd);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::vector<stc_z64> yh((size_t)nfreq * nout * ncol);
+    HIP_TRY(hipMemcpy(yh.data(), Yd, yh.size() * sizeof(stc_z64), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(info, h->info.p, sizeof(int) * nfreq, hipMemcpyDeviceToHost));
+    for (int f = 0; f < nfreq; ++f) {
+        const stc_z64* yf = &yh[(size_t)f * nout * ncol];
+        memcpy(Y + (size_t)f * nout * nrhs, yf, sizeof(stc_z64) * nout * nrhs);
+        if (H) memcpy(H + (size_t)f * nout * nin, yf + (size_t)nout * nrhs, sizeof(stc_z64) * nout * nin);
+    }
+    if (stats) {
+        stats->seconds = ms * 1e-3;
+        stats->points = nfreq;
+    }
+    return status_from_info(info, nfreq, stats);
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+// stc_internal.h -- types shared by the host API and the gfx950 kernels.
+//
+// A launch processes "points": one (system, frequency) pair each.  A system
+// is one ROM (K, M, D, eta, n) with its frequency list, load F and outputs.
+// The descriptor layout is private to the library; the public ABI is
+// include/staccato_sweep.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace stc {
+
+constexpr int kThreads = 256;   // 4 wavefronts of 64 lanes per workgroup
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxGroup = 4;    // right-hand sides solved together
+constexpr int kRedSlots = 1 + 5 * kMaxGroup;  // block reductions in the residual
+// reduction scratch: one row of kRedSlots per wave, the result row, scale factors
+constexpr int kRedDoubles = (kWaves + 1) * kRedSlots + kMaxGroup;
+
+// flags of a launch
+enum : int {
+    kRefine = 1,      // complex64 factors + FP64 refinement
+    kWantBerr = 2,    // compute the backward error (always on when refining)
+    kAdjoint = 4,     // fused adjoint after the forward solve
+    kFactorOut = 8,   // copy factors + pivots out (stc_getrf)
+    kRhsLoad = 16,    // F given per point (the right-hand side for the adjoint pass)
+};
+
+struct SysDesc {
+    int n;
+    int nrhs;
+    int nfreq;
+    int ldf;
+    int ldx;
+    int real_ops;        // K, M, D stored as n*n doubles (imaginary parts zero)
+    int has_D;
+    int p;               // adjoint: number of modal parameters
+    double eta;
+    const double* K;     // n*n (real) or 2*n*n (interleaved complex), ld = n
+    const double* M;
+    const double* D;     // may be null
+    const double* freq;  // Hz
+    const double2* F;    // n x nrhs (ldf)
+    double2* X;          // nfreq blocks of n x nrhs (ldx)
+    int* info;           // nfreq
+    double* berr;        // nfreq * nrhs, nullable
+    int* iters;          // nfreq, nullable (refinement steps used)
+    void* factor_out;    // nfreq blocks of n*n (complex of the factor type), nullable
+    int* ipiv_out;       // nfreq blocks of n, nullable
+    // adjoint
+    const double2* c;    // n
+    const double* Phi;   // n x p, ld = n
+    const double* kdiag; // p
+    double* adj_out;     // nfreq blocks of (p + 2): J_w, dJ/deta_w, dJ/ds_k,w
+    long long item_base; // first point index of this system in the queue
+    // LDS plan chosen by the host (plan_layout) for this system
+    int lay_small;
+    int lay_jb;
+    int lay_cw;
+    int lay_G;
+};
+
+struct KParams {
+    SysDesc one;             // used when sys == nullptr (single system)
+    const SysDesc* sys;      // device array of nsys descriptors, or nullptr
+    int nsys;
+    const int2* items;       // (system, frequency) per point, or nullptr: (0, idx)
+    int nitems;
+    int* counter;            // work-queue head (zeroed by a reset kernel)
+    void* scratch;           // per-workgroup factor slots (blocked path)
+    size_t slot_elems;       // complex elements per slot
+    int flags;
+    int max_iter;
+    double tol;
+    int lds_budget;          // bytes of dynamic LDS given to the launch
+};
+
+// Partition of the dynamic LDS for one point.
+struct Layout {
+    int small;      // 1: whole matrix in LDS; 0: blocked, matrix in a global slot
+    int jb;         // panel width (blocked)
+    int cw;         // trailing column chunk width (blocked)
+    int G;          // right-hand sides per solve group
+    int lds;        // bytes needed
+};
+
+__host__ __device__ inline int align16(long long b) { return (int)((b + 15) & ~15LL); }
+
+// Fixed region at the start of LDS: ipiv, gperm, pperm (n ints each),
+// move pairs (4*64 ints), scalars (16 ints), reduction scratch.
+__host__ __device__ inline int lds_fixed_bytes(int n) {
+    return align16(3LL * n * 4) + align16(4 * 64 * 4) + align16(16 * 4) +
+           align16((long long)kRedDoubles * 8);
+}
+
+// Solve vectors: w (factor type), xs, rs (FP64) and, for the adjoint, the
+// kept forward solution and the adjoint right-hand side (FP64).
+__host__ __device__ inline int lds_solve_bytes(int n, int G, int ts, int adj) {
+    return align16((long long)n * G * ts) + (adj ? 4 : 2) * align16((long long)n * G * 16);
+}
+
+// Choose the path and blocking for a system of order n.  ts = sizeof(complex
+// factor element) (8 or 16).  budget = dynamic LDS bytes of the launch.
+inline Layout plan_layout(int n, int nrhs, int ts, int budget, int jb_cap, int adj) {
+    Layout L{0, 0, 0, 1, 0};
+    const int fixed = lds_fixed_bytes(n);
+    int Gmax = adj ? 1 : (nrhs < kMaxGroup ? nrhs : kMaxGroup);
+    if (Gmax < 1) Gmax = 1;
+    for (int G = Gmax; G >= 1; --G) {
+        int need = fixed + align16((long long)n * n * ts) + lds_solve_bytes(n, G, ts, adj);
+        if (need <= budget) {
+            L.small = 1; L.G = G; L.lds = need;
+            return L;
+        }
+    }
+    const int jbs[] = {64, 48, 32, 28, 24, 20, 16, 12, 8, 4, 2, 1};
+    const int cws[] = {32, 16};
+    for (int a = 0; a < 12; ++a) {
+        int jb = jbs[a];
+        if (jb > jb_cap || jb > n) continue;
+        for (int b = 0; b < 2; ++b) {
+            int cw = cws[b];
+            int need = fixed + align16((long long)n * jb * ts) + align16((long long)jb * jb * ts) +
+                       align16(2LL * jb * cw * ts);
+            if (need <= budget) {
+                L.jb = jb; L.cw = cw; L.lds = need;
+                for (int G = Gmax; G >= 1; --G) {
+                    int ns = fixed + lds_solve_bytes(n, G, ts, adj);
+                    if (ns <= budget) { L.G = G; if (ns > L.lds) L.lds = ns; return L; }
+                }
+                L.lds = 0x7fffffff;  // cannot even hold one solve column
+                return L;
+            }
+        }
+    }
+    L.lds = 0x7fffffff;
+    return L;
+}
+
+}  // namespace stc

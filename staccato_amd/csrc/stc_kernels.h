@@ -1,0 +1,996 @@
+// stc_kernels.h -- gfx950 kernels of the frequency sweep.
+//
+// One workgroup (256 lanes = 4 wavefronts) owns one point (system, frequency)
+// at a time and takes points from a device work queue until it is empty
+// (persistent launch).  Per point it
+//   1. assembles A(w) = (1 + i eta) K + i w D - w^2 M in FP64 from the ROM
+//      operators (KrylovROMSubstructure.cpp:1412-1420) and rounds it to the
+//      factor type T (double: plain path, float: refined path),
+//   2. factors P A = L U with partial pivoting, LAPACK zgetrf semantics
+//      (KrylovROMSubstructure.cpp:1428): pivot = first row of largest
+//      |re| + |im|, full-row interchanges, unit-lower L and U in place,
+//      - small n: the whole matrix lives in LDS, unblocked right-looking;
+//      - larger n: the matrix lives in a per-workgroup slot of global memory
+//        (cache resident) and is factored blocked right-looking: an LDS panel
+//        of jb columns, deferred row interchanges, U12 = L11^{-1} A12 and the
+//        trailing update A22 -= L21 U12 per chunk of columns,
+//   3. solves for all right-hand sides (KrylovROMSubstructure.cpp:1430),
+//   4. optionally refines in FP64: r = F - A x with A re-assembled in FP64,
+//      A d = r with the T factors, x += d, until the normwise backward error
+//      ||r||_inf / (||A||_inf ||x||_inf + ||F||_inf) <= tol,
+//   5. optionally runs the adjoint with the same factors.
+// A(w) is never written to global memory on the small path; on the blocked
+// path the only copy is the workgroup's factor slot.
+#pragma once
+
+#include "stc_internal.h"
+
+namespace stc {
+
+template <typename T>
+struct cx {
+    T re, im;
+};
+using z64 = cx<double>;
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+template <typename T>
+__device__ __forceinline__ cx<T> cmk(T r, T i) {
+    cx<T> z;
+    z.re = r;
+    z.im = i;
+    return z;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> czero() { return cmk<T>(T(0), T(0)); }
+template <typename T>
+__device__ __forceinline__ cx<T> cconj(cx<T> a) { return cmk<T>(a.re, -a.im); }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    return cmk<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+// acc -= a * b
+template <typename T>
+__device__ __forceinline__ void cfms(cx<T>& acc, cx<T> a, cx<T> b) {
+    acc.re = fma(-a.re, b.re, acc.re);
+    acc.re = fma(a.im, b.im, acc.re);
+    acc.im = fma(-a.re, b.im, acc.im);
+    acc.im = fma(-a.im, b.re, acc.im);
+}
+// acc += a * b
+template <typename T>
+__device__ __forceinline__ void cfma(cx<T>& acc, cx<T> a, cx<T> b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+}
+// 1 / z by Smith's method (no overflow of |z|^2)
+template <typename T>
+__device__ __forceinline__ cx<T> crecip(cx<T> z) {
+    if (fabs(z.re) >= fabs(z.im)) {
+        T r = z.im / z.re;
+        T d = z.re + z.im * r;
+        return cmk<T>(T(1) / d, -r / d);
+    }
+    T r = z.re / z.im;
+    T d = z.im + z.re * r;
+    return cmk<T>(r / d, T(-1) / d);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) { return cmul(a, crecip(b)); }
+template <typename T>
+__device__ __forceinline__ T cabs1(cx<T> a) { return fabs(a.re) + fabs(a.im); }
+template <typename T>
+__device__ __forceinline__ cx<T> cshfl(cx<T> z, int src) {
+    return cmk<T>(__shfl(z.re, src, 64), __shfl(z.im, src, 64));
+}
+template <typename T>
+__device__ __forceinline__ cx<T> to_t(z64 a) { return cmk<T>((T)a.re, (T)a.im); }
+template <typename T>
+__device__ __forceinline__ z64 to_z(cx<T> a) { return cmk<double>((double)a.re, (double)a.im); }
+__device__ __forceinline__ z64 ld_z(const double2* p) {
+    double2 v = *p;
+    return cmk<double>(v.x, v.y);
+}
+__device__ __forceinline__ void st_z(double2* p, z64 a) { *p = make_double2(a.re, a.im); }
+// NaN-propagating max
+__device__ __forceinline__ double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
+__device__ __forceinline__ double zabs(z64 a) { return hypot(a.re, a.im); }
+
+// ---- assembly: element e (column-major, ld = n) of A(w) in FP64 ---------------
+__device__ __forceinline__ z64 assemble_at(const SysDesc& s, size_t e, double w, double w2) {
+    if (s.real_ops) {
+        const double k = s.K[e];
+        const double m = s.M[e];
+        const double d = s.has_D ? s.D[e] : 0.0;
+        return cmk<double>(k - w2 * m, s.eta * k + w * d);
+    }
+    const double2 k = reinterpret_cast<const double2*>(s.K)[e];
+    const double2 m = reinterpret_cast<const double2*>(s.M)[e];
+    double2 d = make_double2(0.0, 0.0);
+    if (s.has_D) d = reinterpret_cast<const double2*>(s.D)[e];
+    return cmk<double>(k.x - s.eta * k.y - w2 * m.x - w * d.y,
+                       k.y + s.eta * k.x - w2 * m.y + w * d.x);
+}
+
+// ---- pivot search: first index of max |re|+|im| over rows [start, m) -------------
+// Executed by one whole wavefront; every lane returns the same row.
+template <typename T>
+__device__ __forceinline__ int wave_argmax(const cx<T>* col, int start, int m) {
+    const int lane = threadIdx.x & 63;
+    T best = T(-1);
+    int bi = 0x7fffffff;
+    for (int i = start + lane; i < m; i += 64) {
+        T v = cabs1(col[i]);
+        if (v > best) { best = v; bi = i; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        T ob = __shfl_xor(best, off, 64);
+        int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    return bi == 0x7fffffff ? start : bi;
+}
+
+// ---- unblocked LU of an LDS panel ---------------------------------------------------
+// P: m x jb (column-major, ld ldp) in LDS; local row 0 is global row j0.
+// Two barriers per column: phase S (wave 0 swaps + scales the pivot column,
+// the other waves swap the pivot rows in the remaining panel columns) and
+// phase U (rank-1 update, columns dealt round-robin to the waves; wave 0
+// updates column jj+1 first and searches its pivot, a one-column look-ahead).
+// ipiv[j0+jj] = global 0-based pivot row; gperm / pperm track the composite
+// row permutation (global, and panel-relative); sc[0] = pivot slot,
+// sc[1] = first zero pivot (global, 1-based) or 0.
+template <typename T, int MAXR>
+__device__ void panel_factor(cx<T>* P, int ldp, int m, int jb, int j0, int* ipiv,
+                             int* gperm, int* pperm, int* sc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (wave == 0) {
+        int p = wave_argmax(P, 0, m);
+        if (lane == 0) sc[0] = p;
+    }
+    __syncthreads();
+    for (int jj = 0; jj < jb; ++jj) {
+        const int p = sc[0];
+        cx<T>* colj = P + (size_t)jj * ldp;
+        if (wave == 0) {
+            const cx<T> pv = colj[p];
+            if (pv.re != T(0) || pv.im != T(0)) {
+                const cx<T> r = crecip(pv);
+                cx<T> v[MAXR];
+#pragma unroll
+                for (int t = 0; t < MAXR; ++t) {
+                    const int i = jj + 1 + lane + 64 * t;
+                    if (i < m) v[t] = colj[i == p ? jj : i];
+                }
+#pragma unroll
+                for (int t = 0; t < MAXR; ++t) {
+                    const int i = jj + 1 + lane + 64 * t;
+                    if (i < m) colj[i] = cmul(v[t], r);
+                }
+                if (lane == 0) colj[jj] = pv;
+            } else if (lane == 0 && sc[1] == 0) {
+                sc[1] = j0 + jj + 1;
+            }
+            if (lane == 0) ipiv[j0 + jj] = j0 + p;
+        } else {
+            if (p != jj) {
+                for (int c = tid - 64; c < jb; c += kThreads - 64) {
+                    if (c == jj) continue;
+                    cx<T>* col = P + (size_t)c * ldp;
+                    cx<T> a = col[jj];
+                    col[jj] = col[p];
+                    col[p] = a;
+                }
+                if (tid == 64) {
+                    int t = gperm[j0 + jj];
+                    gperm[j0 + jj] = gperm[j0 + p];
+                    gperm[j0 + p] = t;
+                    if (pperm) {
+                        t = pperm[jj];
+                        pperm[jj] = pperm[p];
+                        pperm[p] = t;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (jj + 1 < jb) {
+            cx<T> l[MAXR];
+#pragma unroll
+            for (int t = 0; t < MAXR; ++t) {
+                const int i = jj + 1 + lane + 64 * t;
+                l[t] = i < m ? colj[i] : czero<T>();
+            }
+            for (int c = jj + 1 + wave; c < jb; c += kWaves) {
+                cx<T>* colc = P + (size_t)c * ldp;
+                const cx<T> u = colc[jj];
+                if (u.re != T(0) || u.im != T(0)) {
+#pragma unroll
+                    for (int t = 0; t < MAXR; ++t) {
+                        const int i = jj + 1 + lane + 64 * t;
+                        if (i < m) {
+                            cx<T> a = colc[i];
+                            cfms(a, l[t], u);
+                            colc[i] = a;
+                        }
+                    }
+                }
+                if (c == jj + 1) {  // wave 0 only: look-ahead pivot search
+                    int pn = wave_argmax(colc, jj + 1, m);
+                    if (lane == 0) sc[0] = pn;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- row interchanges on global columns [c0, c1) via an LDS stash -----------------
+template <typename T>
+__device__ void apply_moves(cx<T>* S, int ld, int c0, int c1, const int* dst, const int* src,
+                            int np, cx<T>* stash, int cws) {
+    if (np == 0) return;
+    const int tid = threadIdx.x;
+    for (int cb = c0; cb < c1; cb += cws) {
+        const int cw = min(cws, c1 - cb);
+        for (int e = tid; e < np * cw; e += kThreads) {
+            const int q = e % np, c = e / np;
+            stash[e] = S[src[q] + (size_t)(cb + c) * ld];
+        }
+        __syncthreads();
+        for (int e = tid; e < np * cw; e += kThreads) {
+            const int q = e % np, c = e / np;
+            S[dst[q] + (size_t)(cb + c) * ld] = stash[e];
+        }
+        __syncthreads();
+    }
+}
+
+// ---- trailing update of one column chunk (VALU) -------------------------------------
+// C = S[j0+jb : j0+m, cb : cb+cw] -= L21 U, L21 = P[jb:m, 0:jb] (LDS, ld ldp),
+// U = jb x cw (LDS, ld jb).  Thread tile: 4 rows x 2 columns.
+template <typename T>
+__device__ void trailing_update(cx<T>* S, int ld, const cx<T>* P, int ldp, int m, int jb,
+                                const cx<T>* U, int j0, int cb, int cw) {
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    for (int r0 = jb; r0 < m; r0 += 64) {
+        cx<T> acc[4][2];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int i = r0 + tx + 16 * a, c = ty + 16 * b;
+                acc[a][b] = (i < m && c < cw) ? S[(size_t)(j0 + i) + (size_t)(cb + c) * ld]
+                                              : czero<T>();
+            }
+        for (int k = 0; k < jb; ++k) {
+            cx<T> l[4], u[2];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int i = min(r0 + tx + 16 * a, m - 1);
+                l[a] = P[i + (size_t)k * ldp];
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int c = min(ty + 16 * b, cw - 1);
+                u[b] = U[k + c * jb];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) cfms(acc[a][b], l[a], u[b]);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int i = r0 + tx + 16 * a, c = ty + 16 * b;
+                if (i < m && c < cw) S[(size_t)(j0 + i) + (size_t)(cb + c) * ld] = acc[a][b];
+            }
+    }
+}
+
+// ---- LDS carve-up ---------------------------------------------------------------------
+struct Lds {
+    int* ipiv;
+    int* gperm;
+    int* pperm;
+    int* pdst;
+    int* psrc;
+    int* sc;
+    double* red;
+    char* main;
+};
+
+__device__ __forceinline__ Lds carve(char* smem, int n) {
+    Lds L;
+    int off = 0;
+    L.ipiv = reinterpret_cast<int*>(smem);
+    L.gperm = L.ipiv + n;
+    L.pperm = L.gperm + n;
+    off += align16(3LL * n * 4);
+    L.pdst = reinterpret_cast<int*>(smem + off);
+    L.psrc = L.pdst + 128;
+    off += align16(4 * 64 * 4);
+    L.sc = reinterpret_cast<int*>(smem + off);
+    off += align16(16 * 4);
+    L.red = reinterpret_cast<double*>(smem + off);
+    off += align16((long long)kRedDoubles * 8);
+    L.main = smem + off;
+    return L;
+}
+
+// ---- blocked factorisation in a global slot ----------------------------------------------
+template <typename T, int MAXR>
+__device__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* S, const Layout& lay,
+                               const Lds& L) {
+    const int n = s.n, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t nn = (size_t)n * n;
+    for (size_t e = tid; e < nn; e += kThreads) S[e] = to_t<T>(assemble_at(s, e, w, w2));
+    for (int r = tid; r < n; r += kThreads) L.gperm[r] = r;
+    if (tid == 0) L.sc[1] = 0;
+    __syncthreads();
+
+    const int jbmax = lay.jb, CW = lay.cw;
+    cx<T>* P = reinterpret_cast<cx<T>*>(L.main);
+    cx<T>* Linv = reinterpret_cast<cx<T>*>(L.main + align16((long long)n * jbmax * sizeof(cx<T>)));
+    cx<T>* Wk = reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(Linv) +
+                                        align16((long long)jbmax * jbmax * sizeof(cx<T>)));
+
+    for (int j0 = 0; j0 < n; j0 += jbmax) {
+        const int jb = min(jbmax, n - j0), m = n - j0;
+        for (int e = tid; e < m * jb; e += kThreads) {
+            const int i = e % m, jj = e / m;
+            P[e] = S[(size_t)(j0 + i) + (size_t)(j0 + jj) * n];
+        }
+        for (int r = tid; r < m; r += kThreads) L.pperm[r] = r;
+        __syncthreads();
+        panel_factor<T, MAXR>(P, m, m, jb, j0, L.ipiv, L.gperm, L.pperm, L.sc);
+        for (int e = tid; e < m * jb; e += kThreads) {
+            const int i = e % m, jj = e / m;
+            S[(size_t)(j0 + i) + (size_t)(j0 + jj) * n] = P[e];
+        }
+        // moved rows of this panel: candidates are the panel rows and the pivot rows
+        if (wave == 0) {
+            int cnt = 0;
+            for (int base = 0; base < 2 * jb; base += 64) {
+                const int q = base + lane;
+                bool keep = false;
+                int r = 0;
+                if (q < 2 * jb) {
+                    r = q < jb ? q : L.ipiv[j0 + q - jb] - j0;
+                    keep = L.pperm[r] != r;
+                    for (int q2 = 0; keep && q2 < q; ++q2) {
+                        const int r2 = q2 < jb ? q2 : L.ipiv[j0 + q2 - jb] - j0;
+                        if (r2 == r) keep = false;
+                    }
+                }
+                const unsigned long long mask = __ballot(keep);
+                const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+                if (keep) {
+                    L.pdst[cnt + pos] = j0 + r;
+                    L.psrc[cnt + pos] = j0 + L.pperm[r];
+                }
+                cnt += __popcll(mask);
+            }
+            if (lane == 0) L.sc[2] = cnt;
+        }
+        // Linv = inverse of the unit-lower L11 (jb <= 64): one column per wave pass
+        for (int j = wave; j < jb; j += kWaves) {
+            cx<T> x = lane == j ? cmk<T>(T(1), T(0)) : czero<T>();
+            for (int k = j; k < jb - 1; ++k) {
+                const cx<T> xk = cshfl(x, k);
+                if (lane > k && lane < jb) cfms(x, P[lane + (size_t)k * m], xk);
+            }
+            if (lane < jb) Linv[lane + j * jb] = x;
+        }
+        __syncthreads();
+        const int np = L.sc[2];
+        apply_moves(S, n, 0, j0, L.pdst, L.psrc, np, Wk, CW);
+        for (int cb = j0 + jb; cb < n; cb += CW) {
+            const int cw = min(CW, n - cb);
+            apply_moves(S, n, cb, cb + cw, L.pdst, L.psrc, np, Wk, CW);
+            for (int e = tid; e < jb * cw; e += kThreads) {
+                const int k = e % jb, c = e / jb;
+                Wk[e] = S[(size_t)(j0 + k) + (size_t)(cb + c) * n];
+            }
+            __syncthreads();
+            cx<T>* U = Wk + jb * CW;
+            for (int e = tid; e < jb * cw; e += kThreads) {
+                const int i = e % jb, c = e / jb;
+                cx<T> acc = Wk[i + c * jb];
+                for (int k = 0; k < i; ++k) cfma(acc, Linv[i + k * jb], Wk[k + c * jb]);
+                U[e] = acc;
+                S[(size_t)(j0 + i) + (size_t)(cb + c) * n] = acc;
+            }
+            __syncthreads();
+            trailing_update(S, n, P, m, m, jb, U, j0, cb, cw);
+            __syncthreads();
+        }
+    }
+}
+
+// ---- triangular solves on LDS vectors w (n x gc, ld n) -----------------------------------
+// L unit lower (in LU), forward.
+template <typename T>
+__device__ void lower_unit_solve(const cx<T>* LU, int ld, int n, cx<T>* w, int gc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        const int kb = min(64, n - k0);
+        for (int g = wave; g < gc; g += kWaves) {
+            cx<T>* y = w + (size_t)g * n + k0;
+            cx<T> v = lane < kb ? y[lane] : czero<T>();
+            const cx<T>* Lb = LU + k0 + (size_t)k0 * ld;
+            for (int k = 0; k < kb - 1; ++k) {
+                const cx<T> vk = cshfl(v, k);
+                if (lane > k && lane < kb) cfms(v, Lb[lane + (size_t)k * ld], vk);
+            }
+            if (lane < kb) y[lane] = v;
+        }
+        __syncthreads();
+        for (int i = k0 + kb + tid; i < n; i += kThreads) {
+            cx<T> acc[kMaxGroup];
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
+            for (int k = 0; k < kb; ++k) {
+                const cx<T> l = LU[i + (size_t)(k0 + k) * ld];
+#pragma unroll
+                for (int g = 0; g < kMaxGroup; ++g)
+                    if (g < gc) cfma(acc[g], l, w[k0 + k + (size_t)g * n]);
+            }
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g)
+                if (g < gc) {
+                    cx<T> a = w[i + (size_t)g * n];
+                    a.re -= acc[g].re;
+                    a.im -= acc[g].im;
+                    w[i + (size_t)g * n] = a;
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// U upper, backward.
+template <typename T>
+__device__ void upper_solve(const cx<T>* LU, int ld, int n, cx<T>* w, int gc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int k1 = n; k1 > 0; k1 -= 64) {
+        const int k0 = max(0, k1 - 64), kb = k1 - k0;
+        for (int g = wave; g < gc; g += kWaves) {
+            cx<T>* y = w + (size_t)g * n + k0;
+            cx<T> v = lane < kb ? y[lane] : czero<T>();
+            const cx<T>* Ub = LU + k0 + (size_t)k0 * ld;
+            for (int k = kb - 1; k >= 0; --k) {
+                if (lane == k) v = cdiv(v, Ub[k + (size_t)k * ld]);
+                const cx<T> vk = cshfl(v, k);
+                if (lane < k) cfms(v, Ub[lane + (size_t)k * ld], vk);
+            }
+            if (lane < kb) y[lane] = v;
+        }
+        __syncthreads();
+        for (int i = tid; i < k0; i += kThreads) {
+            cx<T> acc[kMaxGroup];
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
+            for (int k = 0; k < kb; ++k) {
+                const cx<T> u = LU[i + (size_t)(k0 + k) * ld];
+#pragma unroll
+                for (int g = 0; g < kMaxGroup; ++g)
+                    if (g < gc) cfma(acc[g], u, w[k0 + k + (size_t)g * n]);
+            }
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g)
+                if (g < gc) {
+                    cx<T> a = w[i + (size_t)g * n];
+                    a.re -= acc[g].re;
+                    a.im -= acc[g].im;
+                    w[i + (size_t)g * n] = a;
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// U^H y = b, forward (adjoint).
+template <typename T>
+__device__ void upper_h_solve(const cx<T>* LU, int ld, int n, cx<T>* w, int gc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        const int kb = min(64, n - k0);
+        for (int g = wave; g < gc; g += kWaves) {
+            cx<T>* y = w + (size_t)g * n + k0;
+            cx<T> v = lane < kb ? y[lane] : czero<T>();
+            const cx<T>* Ub = LU + k0 + (size_t)k0 * ld;
+            for (int k = 0; k < kb; ++k) {
+                if (lane == k) v = cdiv(v, cconj(Ub[k + (size_t)k * ld]));
+                const cx<T> vk = cshfl(v, k);
+                if (lane > k && lane < kb) cfms(v, cconj(Ub[k + (size_t)lane * ld]), vk);
+            }
+            if (lane < kb) y[lane] = v;
+        }
+        __syncthreads();
+        for (int i = k0 + kb + tid; i < n; i += kThreads) {
+            cx<T> acc[kMaxGroup];
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
+            for (int k = 0; k < kb; ++k) {
+                const cx<T> u = cconj(LU[(k0 + k) + (size_t)i * ld]);
+#pragma unroll
+                for (int g = 0; g < kMaxGroup; ++g)
+                    if (g < gc) cfma(acc[g], u, w[k0 + k + (size_t)g * n]);
+            }
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g)
+                if (g < gc) {
+                    cx<T> a = w[i + (size_t)g * n];
+                    a.re -= acc[g].re;
+                    a.im -= acc[g].im;
+                    w[i + (size_t)g * n] = a;
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// L^H v = y, backward, unit diagonal (adjoint).
+template <typename T>
+__device__ void lower_h_unit_solve(const cx<T>* LU, int ld, int n, cx<T>* w, int gc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int k1 = n; k1 > 0; k1 -= 64) {
+        const int k0 = max(0, k1 - 64), kb = k1 - k0;
+        for (int g = wave; g < gc; g += kWaves) {
+            cx<T>* y = w + (size_t)g * n + k0;
+            cx<T> v = lane < kb ? y[lane] : czero<T>();
+            const cx<T>* Lb = LU + k0 + (size_t)k0 * ld;
+            for (int k = kb - 1; k > 0; --k) {
+                const cx<T> vk = cshfl(v, k);
+                if (lane < k) cfms(v, cconj(Lb[k + (size_t)lane * ld]), vk);
+            }
+            if (lane < kb) y[lane] = v;
+        }
+        __syncthreads();
+        for (int i = tid; i < k0; i += kThreads) {
+            cx<T> acc[kMaxGroup];
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
+            for (int k = 0; k < kb; ++k) {
+                const cx<T> l = cconj(LU[(k0 + k) + (size_t)i * ld]);
+#pragma unroll
+                for (int g = 0; g < kMaxGroup; ++g)
+                    if (g < gc) cfma(acc[g], l, w[k0 + k + (size_t)g * n]);
+            }
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g)
+                if (g < gc) {
+                    cx<T> a = w[i + (size_t)g * n];
+                    a.re -= acc[g].re;
+                    a.im -= acc[g].im;
+                    w[i + (size_t)g * n] = a;
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- block-wide max of the kRedSlots values (NaN propagating) --------------------------
+__device__ void block_max(double (&vals)[kRedSlots], double* red) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int q = 0; q < kRedSlots; ++q) {
+        double v = vals[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off, 64));
+        vals[q] = v;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < kRedSlots; ++q) red[wave * kRedSlots + q] = vals[q];
+    }
+    __syncthreads();
+    if (tid < kRedSlots) {
+        double v = red[tid];
+        for (int w = 1; w < kWaves; ++w) v = nmax(v, red[w * kRedSlots + tid]);
+        red[kWaves * kRedSlots + tid] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kRedSlots; ++q) vals[q] = red[kWaves * kRedSlots + q];
+}
+
+// ---- block-wide sum of NS values (fixed order, deterministic) -------------------------
+template <int NS>
+__device__ void block_sum(double (&vals)[NS], double* red) {
+    static_assert(NS <= kRedSlots, "reduction scratch");
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        double v = vals[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        vals[q] = v;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) red[wave * kRedSlots + q] = vals[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        double v = red[q];
+        for (int w = 1; w < kWaves; ++w) v += red[w * kRedSlots + q];
+        vals[q] = v;
+    }
+    __syncthreads();
+}
+
+// ---- FP64 residual r = b - op(A) x with the norms of the backward error -----------------
+// ADJ = false: op(A) = A; ADJ = true: op(A) = A^H.  out[0] = ||op(A)||_inf (when
+// need_anorm), then per column g: out[1+4g] = ||r||_inf, out[2+4g] = ||x||_inf,
+// out[3+4g] = ||b||_inf, out[4+4g] = max |re r| + |im r|.
+// b is global (ldb) when bg != nullptr, else LDS (bl, ld n).
+template <bool ADJ>
+__device__ void residual(const SysDesc& s, double w, double w2, const double2* bg, int ldb,
+                         const z64* bl, const z64* xs, z64* rs, int gc, bool need_anorm,
+                         double* red, double (&out)[kRedSlots]) {
+    const int n = s.n, tid = threadIdx.x;
+    double v[kRedSlots];
+#pragma unroll
+    for (int q = 0; q < kRedSlots; ++q) v[q] = 0.0;
+    for (int i = tid; i < n; i += kThreads) {
+        z64 acc[kMaxGroup];
+#pragma unroll
+        for (int g = 0; g < kMaxGroup; ++g)
+            acc[g] = g < gc ? (bg ? ld_z(bg + i + (size_t)g * ldb) : bl[i + (size_t)g * n])
+                            : czero<double>();
+        double asum = 0.0;
+        for (int j = 0; j < n; ++j) {
+            z64 a = ADJ ? cconj(assemble_at(s, (size_t)j + (size_t)i * n, w, w2))
+                        : assemble_at(s, (size_t)i + (size_t)j * n, w, w2);
+            if (need_anorm) asum += zabs(a);
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g)
+                if (g < gc) cfms(acc[g], a, xs[j + (size_t)g * n]);
+        }
+        v[0] = nmax(v[0], asum);
+#pragma unroll
+        for (int g = 0; g < kMaxGroup; ++g)
+            if (g < gc) {
+                rs[i + (size_t)g * n] = acc[g];
+                const z64 b = bg ? ld_z(bg + i + (size_t)g * ldb) : bl[i + (size_t)g * n];
+                v[1 + 4 * g] = nmax(v[1 + 4 * g], zabs(acc[g]));
+                v[2 + 4 * g] = nmax(v[2 + 4 * g], zabs(xs[i + (size_t)g * n]));
+                v[3 + 4 * g] = nmax(v[3 + 4 * g], zabs(b));
+                v[4 + 4 * g] = nmax(v[4 + 4 * g], cabs1(acc[g]));
+            }
+    }
+    block_max(v, red);
+#pragma unroll
+    for (int q = 0; q < kRedSlots; ++q) out[q] = v[q];
+}
+
+__device__ __forceinline__ double backward_error(double rmax, double anorm, double xmax,
+                                                 double bmax) {
+    const double den = anorm * xmax + bmax;
+    if (den == 0.0) return rmax == 0.0 ? 0.0 : __builtin_inf();
+    return rmax / den;
+}
+
+// ---- solve (+ refine) one group of right-hand sides ---------------------------------------
+// On entry wv holds P b (permuted, type T) for gc columns and xs/rs are free.
+// On exit xs holds the solution (FP64).  Returns refinement steps; conv[] = 1
+// for columns meeting tol; berr_out[] their backward errors (when computed).
+template <typename T, bool ADJ>
+__device__ int solve_group(const SysDesc& s, double w, double w2, const cx<T>* LU, int ld,
+                           const int* gperm, cx<T>* wv, z64* xs, z64* rs, int gc,
+                           const double2* bg, int ldb, const z64* bl, bool refine, bool want_berr,
+                           double tol, int max_iter, double* anorm,
+                           double (&berr_out)[kMaxGroup], bool* allconv, double* red) {
+    const int n = s.n, tid = threadIdx.x;
+    auto solve_t = [&]() {
+        if (!ADJ) {
+            lower_unit_solve(LU, ld, n, wv, gc);
+            upper_solve(LU, ld, n, wv, gc);
+        } else {
+            upper_h_solve(LU, ld, n, wv, gc);
+            lower_h_unit_solve(LU, ld, n, wv, gc);
+        }
+    };
+    // x = op(A)^{-1} b in T; ADJ leaves the result un-permuted (x[gperm[r]] = wv[r])
+    solve_t();
+    for (int e = tid; e < n * gc; e += kThreads) {
+        const int r = e % n, g = e / n;
+        const int dst = ADJ ? gperm[r] : r;
+        xs[dst + (size_t)g * n] = to_z(wv[e]);
+    }
+    __syncthreads();
+    int it = 0;
+    *allconv = true;
+    if (!want_berr) return 0;
+    for (;;) {
+        double out[kRedSlots];
+        residual<ADJ>(s, w, w2, bg, ldb, bl, xs, rs, gc, *anorm < 0.0, red, out);
+        if (*anorm < 0.0) *anorm = out[0];
+        bool conv = true;
+#pragma unroll
+        for (int g = 0; g < kMaxGroup; ++g) {
+            if (g < gc) {
+                const double be =
+                    backward_error(out[1 + 4 * g], *anorm, out[2 + 4 * g], out[3 + 4 * g]);
+                berr_out[g] = be;
+                if (!(be <= tol)) conv = false;
+            }
+        }
+        if (!refine || conv || it >= max_iter) {
+            *allconv = conv;
+            break;
+        }
+        // d = op(A)^{-1} (r / s) in T, x += s d  (s = max |re|+|im| of r per column)
+        double* scl = red + (kWaves + 1) * kRedSlots;
+        if (tid < kMaxGroup) {
+            double sv = 1.0;
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g)
+                if (g == tid && out[4 + 4 * g] > 0.0) sv = out[4 + 4 * g];
+            scl[tid] = sv;
+        }
+        __syncthreads();
+        for (int e = tid; e < n * gc; e += kThreads) {
+            const int r = e % n, g = e / n;
+            const double sg = scl[g];
+            const z64 rv = rs[(ADJ ? r : gperm[r]) + (size_t)g * n];
+            wv[e] = to_t<T>(cmk<double>(rv.re / sg, rv.im / sg));
+        }
+        __syncthreads();
+        solve_t();
+        for (int e = tid; e < n * gc; e += kThreads) {
+            const int r = e % n, g = e / n;
+            const double sg = scl[g];
+            const int dst = ADJ ? gperm[r] : r;
+            z64 x = xs[dst + (size_t)g * n];
+            const z64 d = to_z(wv[e]);
+            x.re = fma(sg, d.re, x.re);
+            x.im = fma(sg, d.im, x.im);
+            xs[dst + (size_t)g * n] = x;
+        }
+        __syncthreads();
+        ++it;
+    }
+    return it;
+}
+
+// ---- fused adjoint of one point ------------------------------------------------------------
+// J_w = sum_r |c^H x_r|^2; mu_r = A^{-H} (c g_r) with the factors already in
+// LU; dJ/deta_w = -2 Re(mu^H (i K) x) and dJ/ds_k,w = -2 Re((1+i eta) kdiag_k
+// conj(phi_k^T mu) (phi_k^T x)), summed over r.  Writes adj_out[fidx*(p+2) + .]
+// = {J_w, dJ/deta_w, dJ/ds_0,w, ...} and info[fidx] (0, or -1 when a refined
+// solve missed tol).
+template <typename T>
+__device__ void adjoint_point(const SysDesc& s, int fidx, double w, double w2, const cx<T>* LU,
+                              const Lds& L, cx<T>* wv, z64* xs, z64* rs, z64* xk, z64* bv,
+                              bool refine, double tol, int max_iter) {
+    const int n = s.n, tid = threadIdx.x, p = s.p;
+    double* out = s.adj_out + (size_t)fidx * (p + 2);
+    double anorm_f = -1.0, anorm_a = -1.0;
+    double accJ = 0.0, accE = 0.0;
+    bool ok = true;
+    for (int r = 0; r < s.nrhs; ++r) {
+        const double2* Fr = s.F + (size_t)r * s.ldf;
+        for (int i = tid; i < n; i += kThreads) wv[i] = to_t<T>(ld_z(Fr + L.gperm[i]));
+        __syncthreads();
+        double be[kMaxGroup];
+        bool conv;
+        solve_group<T, false>(s, w, w2, LU, n, L.gperm, wv, xs, rs, 1, Fr, s.ldf, nullptr, refine,
+                              refine, tol, max_iter, &anorm_f, be, &conv, L.red);
+        if (!conv) ok = false;
+        // g = c^H x
+        double gs[2] = {0.0, 0.0};
+        for (int i = tid; i < n; i += kThreads) {
+            const z64 x = xs[i];
+            xk[i] = x;
+            const z64 c = ld_z(s.c + i);
+            gs[0] += c.re * x.re + c.im * x.im;
+            gs[1] += c.re * x.im - c.im * x.re;
+        }
+        block_sum<2>(gs, L.red);
+        const z64 g = cmk<double>(gs[0], gs[1]);
+        if (s.X) {
+            double2* X = s.X + (size_t)fidx * s.ldx * s.nrhs + (size_t)r * s.ldx;
+            for (int i = tid; i < n; i += kThreads) st_z(X + i, xk[i]);
+        }
+        // adjoint right-hand side b = c g (not permuted for the A^H solve)
+        for (int i = tid; i < n; i += kThreads) {
+            const z64 b = cmul(ld_z(s.c + i), g);
+            bv[i] = b;
+            wv[i] = to_t<T>(b);
+        }
+        __syncthreads();
+        solve_group<T, true>(s, w, w2, LU, n, L.gperm, wv, xs, rs, 1, nullptr, n, bv, refine,
+                             refine, tol, max_iter, &anorm_a, be, &conv, L.red);
+        if (!conv) ok = false;
+        // mu^H (K x)
+        double ks[2] = {0.0, 0.0};
+        for (int i = tid; i < n; i += kThreads) {
+            z64 kx = czero<double>();
+            for (int j = 0; j < n; ++j) {
+                const size_t e = (size_t)i + (size_t)j * n;
+                z64 kij;
+                if (s.real_ops) kij = cmk<double>(s.K[e], 0.0);
+                else kij = ld_z(reinterpret_cast<const double2*>(s.K) + e);
+                cfma(kx, kij, xk[j]);
+            }
+            cfma(*reinterpret_cast<z64*>(ks), cconj(xs[i]), kx);
+        }
+        block_sum<2>(ks, L.red);
+        // -2 Re(i * s) = 2 Im(s)
+        accE += 2.0 * ks[1];
+        accJ += g.re * g.re + g.im * g.im;
+        const z64 keta = cmk<double>(1.0, s.eta);
+        for (int k = tid; k < p; k += kThreads) {
+            const double* ph = s.Phi + (size_t)k * n;
+            z64 pm = czero<double>(), px = czero<double>();
+            for (int i = 0; i < n; ++i) {
+                const double f = ph[i];
+                pm.re = fma(f, xs[i].re, pm.re);
+                pm.im = fma(f, xs[i].im, pm.im);
+                px.re = fma(f, xk[i].re, px.re);
+                px.im = fma(f, xk[i].im, px.im);
+            }
+            const z64 t = cmul(cmul(keta, cconj(pm)), px);
+            const double d = -2.0 * s.kdiag[k] * t.re;
+            out[2 + k] = (r == 0) ? d : out[2 + k] + d;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out[0] = accJ;
+        out[1] = accE;
+        s.info[fidx] = ok ? 0 : -1;
+        if (s.iters) s.iters[fidx] = 0;
+    }
+    __syncthreads();
+}
+
+// ---- one point ---------------------------------------------------------------------------
+template <typename T, int MAXR>
+__device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, char* smem) {
+    const int n = s.n, tid = threadIdx.x, nrhs = s.nrhs;
+    const double w = kTwoPi * s.freq[fidx], w2 = w * w;
+    Layout lay;
+    lay.small = s.lay_small;
+    lay.jb = s.lay_jb;
+    lay.cw = s.lay_cw;
+    lay.G = s.lay_G;
+    lay.lds = 0;
+    const Lds L = carve(smem, n);
+    const bool refine = (kp.flags & kRefine) != 0;
+    const bool want_berr = refine || (kp.flags & kWantBerr) || (s.berr != nullptr);
+
+    const cx<T>* LU;
+    char* solvep;
+    if (lay.small) {
+        cx<T>* A = reinterpret_cast<cx<T>*>(L.main);
+        const size_t nn = (size_t)n * n;
+        for (size_t e = tid; e < nn; e += kThreads) A[e] = to_t<T>(assemble_at(s, e, w, w2));
+        for (int r = tid; r < n; r += kThreads) L.gperm[r] = r;
+        if (tid == 0) L.sc[1] = 0;
+        __syncthreads();
+        panel_factor<T, MAXR>(A, n, n, n, 0, L.ipiv, L.gperm, nullptr, L.sc);
+        LU = A;
+        solvep = L.main + align16((long long)nn * sizeof(cx<T>));
+    } else {
+        cx<T>* S = reinterpret_cast<cx<T>*>(kp.scratch) + (size_t)blockIdx.x * kp.slot_elems;
+        if (kp.flags & kFactorOut)
+            S = reinterpret_cast<cx<T>*>(s.factor_out) + (size_t)fidx * n * n;
+        factor_blocked<T, MAXR>(s, w, w2, S, lay, L);
+        LU = S;
+        solvep = L.main;
+    }
+    const int info = L.sc[1];
+
+    if (kp.flags & kFactorOut) {
+        if (lay.small) {
+            cx<T>* out = reinterpret_cast<cx<T>*>(s.factor_out) + (size_t)fidx * n * n;
+            for (size_t e = tid; e < (size_t)n * n; e += kThreads) out[e] = LU[e];
+        }
+        for (int r = tid; r < n; r += kThreads) s.ipiv_out[(size_t)fidx * n + r] = L.ipiv[r] + 1;
+        if (tid == 0) s.info[fidx] = info;
+        __syncthreads();
+        return;
+    }
+
+    double2* X = s.X + (size_t)fidx * s.ldx * nrhs;
+    if (info != 0) {
+        const double qnan = __builtin_nan("");
+        for (int e = tid; e < n * nrhs; e += kThreads) {
+            const int r = e % n, g = e / n;
+            X[r + (size_t)g * s.ldx] = make_double2(qnan, qnan);
+        }
+        if (s.berr)
+            for (int g = tid; g < nrhs; g += kThreads) s.berr[(size_t)fidx * nrhs + g] = __builtin_inf();
+        if (tid == 0) {
+            s.info[fidx] = info;
+            if (s.iters) s.iters[fidx] = 0;
+        }
+        __syncthreads();
+        return;
+    }
+
+    const int G = lay.G;
+    cx<T>* wv = reinterpret_cast<cx<T>*>(solvep);
+    z64* xs = reinterpret_cast<z64*>(solvep + align16((long long)n * G * sizeof(cx<T>)));
+    z64* rs = reinterpret_cast<z64*>(reinterpret_cast<char*>(xs) + align16((long long)n * G * 16));
+    if (kp.flags & kAdjoint) {
+        z64* xk = reinterpret_cast<z64*>(reinterpret_cast<char*>(rs) + align16((long long)n * G * 16));
+        z64* bv = reinterpret_cast<z64*>(reinterpret_cast<char*>(xk) + align16((long long)n * G * 16));
+        adjoint_point<T>(s, fidx, w, w2, LU, L, wv, xs, rs, xk, bv, refine, kp.tol, kp.max_iter);
+        return;
+    }
+    double anorm = -1.0;
+    int it_max = 0;
+    bool ok = true;
+    for (int g0 = 0; g0 < nrhs; g0 += G) {
+        const int gc = min(G, nrhs - g0);
+        const double2* Fg = s.F + (size_t)g0 * s.ldf;
+        for (int e = tid; e < n * gc; e += kThreads) {
+            const int r = e % n, g = e / n;
+            wv[e] = to_t<T>(ld_z(Fg + L.gperm[r] + (size_t)g * s.ldf));
+        }
+        __syncthreads();
+        double be[kMaxGroup];
+        bool conv;
+        const int it = solve_group<T, false>(s, w, w2, LU, n, L.gperm, wv, xs, rs, gc, Fg, s.ldf,
+                                             nullptr, refine, want_berr, kp.tol, kp.max_iter,
+                                             &anorm, be, &conv, L.red);
+        if (refine && !conv) ok = false;
+        if (it > it_max) it_max = it;
+        for (int e = tid; e < n * gc; e += kThreads) {
+            const int r = e % n, g = e / n;
+            st_z(X + r + (size_t)(g0 + g) * s.ldx, xs[e]);
+        }
+        if (s.berr && want_berr && tid == 0) {
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g)
+                if (g < gc) s.berr[(size_t)fidx * nrhs + g0 + g] = be[g];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        s.info[fidx] = ok ? 0 : -1;
+        if (s.iters) s.iters[fidx] = it_max;
+    }
+}
+
+// ---- the persistent kernel -------------------------------------------------------------
+template <typename T, int MAXR>
+__global__ void __launch_bounds__(kThreads) sweep_kernel(KParams kp) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    __shared__ int s_item;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(kp.counter, 1);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();
+        if (item >= kp.nitems) break;
+        int sysi = 0, fidx = item;
+        if (kp.items) {
+            const int2 it = kp.items[item];
+            sysi = it.x;
+            fidx = it.y;
+        }
+        const SysDesc s = kp.sys ? kp.sys[sysi] : kp.one;
+        process_point<T, MAXR>(s, fidx, kp, smem);
+        __syncthreads();
+    }
+}
+
+__global__ void reset_counter_kernel(int* counter) {
+    if (threadIdx.x == 0) counter[0] = 0;
+}
+
+}  // namespace stc

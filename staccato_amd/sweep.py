@@ -1,0 +1,228 @@
+"""Python host interface of the MI355X frequency sweep (over the C ABI).
+
+Mirrors the reference's sweep entry (KrylovROMSubstructure::backTransformKMOR,
+src/analysis/KrylovROMSubstructure.cpp:1393-1460): the ROM operators are set
+once, then every call solves A(w) X = F for a list of frequencies in Hz
+(w = 2 pi f, :1408).  Arrays are numpy complex128; matrices are passed to the
+library column-major (the layout of MKL_Complex16 arrays in the reference).
+Errors come back as StcError with the library status, never as exit().
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import STC_PLAIN_FP64, STC_REFINED_FP32, Stats, SysDesc
+
+MODES = {"plain": STC_PLAIN_FP64, "fp64": STC_PLAIN_FP64,
+         "refined": STC_REFINED_FP32, "fp32": STC_REFINED_FP32}
+
+
+class StcError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {_lib.strerror(code)} ({code})")
+        self.code = code
+
+
+def _mode(mode):
+    if isinstance(mode, int):
+        return mode
+    return MODES[mode]
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _fortran_c128(a, ndim=2):
+    a = np.asarray(a, dtype=np.complex128)
+    if a.ndim == 1 and ndim == 2:
+        a = a[:, None]
+    return np.asfortranarray(a)
+
+
+def _check(rc, what, soft=(0,)):
+    if rc not in soft:
+        raise StcError(rc, what)
+    return rc
+
+
+class Handle:
+    """One device, one stream, a set of ROMs (stc_create / stc_destroy)."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        _check(self.lib.stc_create(ctypes.byref(h), int(device)), "stc_create")
+        self.h = h
+        self.n = {}
+
+    def close(self):
+        if self.h:
+            self.lib.stc_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_ok(self):
+        return bool(self.lib.stc_device_ok(self.h))
+
+    # ---- operators --------------------------------------------------------------
+    def rom_set(self, rom_id, K, M, D=None, eta=0.0):
+        K = _fortran_c128(K)
+        M = _fortran_c128(M)
+        Dd = _fortran_c128(D) if D is not None else None
+        n = K.shape[0]
+        _check(self.lib.stc_rom_set(self.h, rom_id, n, _ptr(K), _ptr(M), _ptr(Dd), n,
+                                    float(eta)), "stc_rom_set")
+        self.n[rom_id] = n
+
+    def rom_set_io(self, rom_id, B, C):
+        B = _fortran_c128(B)
+        C = _fortran_c128(C)
+        _check(self.lib.stc_rom_set_io(self.h, rom_id, B.shape[1], _ptr(B), B.shape[0],
+                                       C.shape[0], _ptr(C), C.shape[0]), "stc_rom_set_io")
+
+    def rom_clear(self, rom_id):
+        _check(self.lib.stc_rom_clear(self.h, rom_id), "stc_rom_clear")
+        self.n.pop(rom_id, None)
+
+    # ---- sweep ------------------------------------------------------------------
+    def sweep(self, rom_id, freq, F, mode="plain", tol=0.0, max_iter=0, berr=True):
+        """X[j] = A(2 pi freq[j])^{-1} F.  Returns (X (nfreq, n, nrhs), info, berr, stats)."""
+        n = self.n[rom_id]
+        freq = np.ascontiguousarray(freq, dtype=np.float64)
+        F = _fortran_c128(F)
+        nf, nrhs = freq.size, F.shape[1]
+        X = np.empty((nf, nrhs, n), dtype=np.complex128)
+        info = np.zeros(nf, dtype=np.int32)
+        be = np.empty((nf, nrhs), dtype=np.float64) if berr else None
+        st = Stats()
+        rc = self.lib.stc_sweep(self.h, rom_id, nf, _ptr(freq), nrhs, _ptr(F), F.shape[0],
+                                _ptr(X), n, _ptr(info), _ptr(be), _mode(mode), float(tol),
+                                int(max_iter), ctypes.byref(st))
+        _check(rc, "stc_sweep", soft=(0, 1, 2))
+        return X.transpose(0, 2, 1), info, be, st
+
+    def sweep_dev(self, rom_id, freq_t, F_t, X_t, info_t, berr_t=None, mode="plain", tol=0.0,
+                  max_iter=0, stream=None):
+        """Device-resident sweep on torch tensors (no copies, no synchronisation).
+        F_t: (nrhs, n) complex128 contiguous (column-major n x nrhs), X_t:
+        (nfreq, nrhs, n) complex128, info_t int32 (nfreq), berr_t float64 or None."""
+        n = self.n[rom_id]
+        nrhs = F_t.shape[0]
+        s = ctypes.c_void_p(stream) if stream else None
+        rc = self.lib.stc_sweep_dev(
+            self.h, rom_id, freq_t.numel(), ctypes.c_void_p(freq_t.data_ptr()), nrhs,
+            ctypes.c_void_p(F_t.data_ptr()), n, ctypes.c_void_p(X_t.data_ptr()), n,
+            ctypes.c_void_p(info_t.data_ptr()),
+            ctypes.c_void_p(berr_t.data_ptr()) if berr_t is not None else None,
+            _mode(mode), float(tol), int(max_iter), s)
+        _check(rc, "stc_sweep_dev")
+
+    def getrf(self, rom_id, freq, mode="plain"):
+        """LU factors (LAPACK layout, nfreq x n x n), 1-based pivots, info."""
+        n = self.n[rom_id]
+        freq = np.ascontiguousarray(freq, dtype=np.float64)
+        nf = freq.size
+        dt = np.complex128 if _mode(mode) == STC_PLAIN_FP64 else np.complex64
+        LU = np.empty((nf, n, n), dtype=dt)       # [f, col, row] in memory
+        ipiv = np.empty((nf, n), dtype=np.int32)
+        info = np.zeros(nf, dtype=np.int32)
+        rc = self.lib.stc_getrf(self.h, rom_id, nf, _ptr(freq), _mode(mode), _ptr(LU),
+                                _ptr(ipiv), _ptr(info))
+        _check(rc, "stc_getrf", soft=(0, 1))
+        return LU.transpose(0, 2, 1), ipiv, info
+
+    def sweep_batch(self, systems, mode="refined", tol=0.0, max_iter=0):
+        """systems: list of (rom_id, freq, F).  One persistent launch over all
+        points.  Returns list of (X, info, berr) and the stats."""
+        keep, descs, outs = [], (SysDesc * len(systems))(), []
+        for i, (rid, freq, F) in enumerate(systems):
+            n = self.n[rid]
+            freq = np.ascontiguousarray(freq, dtype=np.float64)
+            F = _fortran_c128(F)
+            nf, nrhs = freq.size, F.shape[1]
+            X = np.empty((nf, nrhs, n), dtype=np.complex128)
+            info = np.zeros(nf, dtype=np.int32)
+            be = np.empty((nf, nrhs), dtype=np.float64)
+            keep += [freq, F]
+            outs.append((X, info, be))
+            d = descs[i]
+            d.rom_id, d.nfreq, d.freq_hz, d.nrhs = rid, nf, freq.ctypes.data, nrhs
+            d.F, d.ldf, d.X, d.ldx = F.ctypes.data, F.shape[0], X.ctypes.data, n
+            d.info, d.berr = info.ctypes.data, be.ctypes.data
+        st = Stats()
+        rc = self.lib.stc_sweep_batch(self.h, len(systems), descs, _mode(mode), float(tol),
+                                      int(max_iter), ctypes.byref(st))
+        _check(rc, "stc_sweep_batch", soft=(0, 1, 2))
+        return [(X.transpose(0, 2, 1), info, be) for X, info, be in outs], st
+
+    def adjoint(self, rom_id, freq, F, c, Phi, kdiag, mode="plain", tol=0.0, max_iter=0):
+        """J = sum_w sum_r |c^H X_r(w)|^2 and dJ/ds (modal stiffness scales), dJ/deta."""
+        n = self.n[rom_id]
+        freq = np.ascontiguousarray(freq, dtype=np.float64)
+        F = _fortran_c128(F)
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        Phi = np.asfortranarray(Phi, dtype=np.float64)
+        kdiag = np.ascontiguousarray(kdiag, dtype=np.float64)
+        p = Phi.shape[1]
+        J, de = ctypes.c_double(), ctypes.c_double()
+        ds = np.zeros(p, dtype=np.float64)
+        info = np.zeros(freq.size, dtype=np.int32)
+        rc = self.lib.stc_adjoint(self.h, rom_id, freq.size, _ptr(freq), F.shape[1], _ptr(F),
+                                  n, _ptr(c), p, _ptr(Phi), n, _ptr(kdiag), _mode(mode),
+                                  float(tol), int(max_iter), ctypes.byref(J), _ptr(ds),
+                                  ctypes.byref(de), _ptr(info))
+        _check(rc, "stc_adjoint", soft=(0, 1, 2))
+        return J.value, ds, de.value, info
+
+    def back_transform(self, rom_id, freq, input_load, nin, nout, transfer=True, mode="plain",
+                       tol=0.0, max_iter=0):
+        """y(w) = C_R A(w)^{-1} B_R inputLoad (nfreq, nout, nrhs) and, when
+        transfer, H(w) = C_R A(w)^{-1} B_R (nfreq, nout, nin)."""
+        freq = np.ascontiguousarray(freq, dtype=np.float64)
+        L = _fortran_c128(input_load)
+        nrhs = L.shape[1]
+        nf = freq.size
+        Y = np.empty((nf, nrhs, nout), dtype=np.complex128)
+        H = np.empty((nf, nin, nout), dtype=np.complex128) if transfer else None
+        info = np.zeros(nf, dtype=np.int32)
+        st = Stats()
+        rc = self.lib.stc_back_transform(self.h, rom_id, nf, _ptr(freq), _ptr(L), nrhs, _ptr(Y),
+                                         _ptr(H), _ptr(info), _mode(mode), float(tol),
+                                         int(max_iter), ctypes.byref(st))
+        _check(rc, "stc_back_transform", soft=(0, 1, 2))
+        return (Y.transpose(0, 2, 1), H.transpose(0, 2, 1) if transfer else None, info, st)
+
+
+class Plan:
+    """A captured (hipGraph) persistent launch over device-resident systems."""
+
+    def __init__(self, handle, descs, mode="refined", tol=0.0, max_iter=0):
+        self.handle = handle
+        self.descs = descs
+        p = ctypes.c_void_p()
+        _check(handle.lib.stc_plan_create(handle.h, len(descs), descs, _mode(mode), float(tol),
+                                          int(max_iter), ctypes.byref(p)), "stc_plan_create")
+        self.p = p
+
+    def execute(self, stream=None):
+        _check(self.handle.lib.stc_plan_execute(self.p, ctypes.c_void_p(stream) if stream else None),
+               "stc_plan_execute")
+
+    def close(self):
+        if self.p:
+            self.handle.lib.stc_plan_destroy(self.p)
+            self.p = None
