@@ -185,10 +185,12 @@ int configure(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int refin
     long long g = (long long)h->num_cu * per_cu;
     if (const char* e = getenv("STC_GRID")) g = std::max(1, atoi(e));
     cfg->grid = (int)std::max(1LL, std::min(g, nitems));
+    // every blocked system factors in a per-workgroup slot (also for stc_getrf,
+    // which copies the slot out in LAPACK layout afterwards)
+    (void)factor_out;
     size_t slot = 0;
-    if (!factor_out)
-        for (const SysDesc& s : sys)
-            if (!s.lay_small) slot = std::max(slot, (size_t)s.n * s.n);
+    for (const SysDesc& s : sys)
+        if (!s.lay_small) slot = std::max(slot, (size_t)s.n * s.n);
     cfg->slot_elems = slot;
     if (slot) {
         rc = h->scratch.ensure(slot * ts * (size_t)cfg->grid);

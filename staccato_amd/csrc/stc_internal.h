@@ -88,11 +88,14 @@ struct Layout {
 
 __host__ __device__ inline int align16(long long b) { return (int)((b + 15) & ~15LL); }
 
-// Fixed region at the start of LDS: ipiv, gperm, pperm (n ints each),
-// move pairs (4*64 ints), scalars (16 ints), reduction scratch.
+constexpr int kChunkCols = 32;    // trailing columns per chunk (kernel: kChunk)
+constexpr int kChunkLdCols = 34;  // padded row length of the U12 chunk in LDS
+constexpr int kSolveBlk = 32;     // diagonal block of the blocked solves
+
+// Fixed region at the start of LDS: ipiv and the row map (n ints each),
+// scalars (16 ints), reduction scratch.
 __host__ __device__ inline int lds_fixed_bytes(int n) {
-    return align16(3LL * n * 4) + align16(4 * 64 * 4) + align16(16 * 4) +
-           align16((long long)kRedDoubles * 8);
+    return align16(2LL * n * 4) + align16(16 * 4) + align16((long long)kRedDoubles * 8);
 }
 
 // Solve vectors: w (factor type), xs, rs (FP64) and, for the adjoint, the
@@ -115,24 +118,22 @@ inline Layout plan_layout(int n, int nrhs, int ts, int budget, int jb_cap, int a
             return L;
         }
     }
-    const int jbs[] = {64, 48, 32, 28, 24, 20, 16, 12, 8, 4, 2, 1};
-    const int cws[] = {32, 16};
-    for (int a = 0; a < 12; ++a) {
+    // blocked: LDS panel (n x jb) + one U12 chunk (jb x kChunkLdCols); the
+    // solve phase reuses the same region for its vectors + a staged diagonal block
+    const int jbs[] = {64, 48, 32, 28, 24, 20, 16, 12, 8, 4};
+    for (int a = 0; a < 10; ++a) {
         int jb = jbs[a];
-        if (jb > jb_cap || jb > n) continue;
-        for (int b = 0; b < 2; ++b) {
-            int cw = cws[b];
-            int need = fixed + align16((long long)n * jb * ts) + align16((long long)jb * jb * ts) +
-                       align16(2LL * jb * cw * ts);
-            if (need <= budget) {
-                L.jb = jb; L.cw = cw; L.lds = need;
-                for (int G = Gmax; G >= 1; --G) {
-                    int ns = fixed + lds_solve_bytes(n, G, ts, adj);
-                    if (ns <= budget) { L.G = G; if (ns > L.lds) L.lds = ns; return L; }
-                }
-                L.lds = 0x7fffffff;  // cannot even hold one solve column
-                return L;
+        if (jb > jb_cap) continue;
+        int need = fixed + align16((long long)n * jb * ts) + align16((long long)jb * kChunkLdCols * ts);
+        if (need <= budget) {
+            L.jb = jb; L.cw = kChunkCols; L.lds = need;
+            for (int G = Gmax; G >= 1; --G) {
+                int ns = fixed + lds_solve_bytes(n, G, ts, adj) +
+                         align16((long long)kSolveBlk * kSolveBlk * ts);
+                if (ns <= budget) { L.G = G; if (ns > L.lds) L.lds = ns; return L; }
             }
+            L.lds = 0x7fffffff;  // cannot even hold one solve column
+            return L;
         }
     }
     L.lds = 0x7fffffff;

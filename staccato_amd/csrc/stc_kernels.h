@@ -141,12 +141,12 @@ __device__ __forceinline__ int wave_argmax(const cx<T>* col, int start, int m) {
 // the other waves swap the pivot rows in the remaining panel columns) and
 // phase U (rank-1 update, columns dealt round-robin to the waves; wave 0
 // updates column jj+1 first and searches its pivot, a one-column look-ahead).
-// ipiv[j0+jj] = global 0-based pivot row; gperm / pperm track the composite
-// row permutation (global, and panel-relative); sc[0] = pivot slot,
-// sc[1] = first zero pivot (global, 1-based) or 0.
+// ipiv[j0+jj] = global 0-based pivot row; rmap (logical row -> physical row,
+// the composite row permutation) gets the same interchange; sc[0] = pivot
+// slot, sc[1] = first zero pivot (global, 1-based) or 0.
 template <typename T, int MAXR>
 __device__ void panel_factor(cx<T>* P, int ldp, int m, int jb, int j0, int* ipiv,
-                             int* gperm, int* pperm, int* sc) {
+                             int* rmap, int* sc) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (wave == 0) {
         int p = wave_argmax(P, 0, m);
@@ -186,14 +186,9 @@ __device__ void panel_factor(cx<T>* P, int ldp, int m, int jb, int j0, int* ipiv
                     col[p] = a;
                 }
                 if (tid == 64) {
-                    int t = gperm[j0 + jj];
-                    gperm[j0 + jj] = gperm[j0 + p];
-                    gperm[j0 + p] = t;
-                    if (pperm) {
-                        t = pperm[jj];
-                        pperm[jj] = pperm[p];
-                        pperm[p] = t;
-                    }
+                    const int t = rmap[j0 + jj];
+                    rmap[j0 + jj] = rmap[j0 + p];
+                    rmap[j0 + p] = t;
                 }
             }
         }
@@ -229,80 +224,198 @@ __device__ void panel_factor(cx<T>* P, int ldp, int m, int jb, int j0, int* ipiv
     }
 }
 
-// ---- row interchanges on global columns [c0, c1) via an LDS stash -----------------
+// ---- MFMA 16x16x4 (one wavefront): D[i][j] += sum_k A[i][k] B[k][j] ------------------
+// Lane l supplies A[l%16][l/16] and B[l/16][l%16]; D[4*(l/16)+r][l%16] is in
+// accumulator element r.
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
 template <typename T>
-__device__ void apply_moves(cx<T>* S, int ld, int c0, int c1, const int* dst, const int* src,
-                            int np, cx<T>* stash, int cws) {
-    if (np == 0) return;
-    const int tid = threadIdx.x;
-    for (int cb = c0; cb < c1; cb += cws) {
-        const int cw = min(cws, c1 - cb);
-        for (int e = tid; e < np * cw; e += kThreads) {
-            const int q = e % np, c = e / np;
-            stash[e] = S[src[q] + (size_t)(cb + c) * ld];
+struct Mfma;
+template <>
+struct Mfma<float> {
+    using acc = v4f;
+    static __device__ __forceinline__ acc run(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma<double> {
+    using acc = v4d;
+    static __device__ __forceinline__ acc run(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int kChunk = 32;      // trailing columns per TRSM/update chunk
+constexpr int kChunkLd = kChunk + 2;
+constexpr int kSolveBlock = 32; // diagonal block of the blocked triangular solves
+
+// ---- U12 chunk = L11^{-1} A12 (wave substitution) -----------------------------------------
+// A12: logical rows [j0, j0+jb) of global columns [cb, cb+cw) (row r lives at
+// physical row rmap[r]); L11 = unit-lower P[0:jb, 0:jb] (LDS, ld ldp).  Each
+// wave solves columns wave, wave+4, ... (lanes = rows, jb <= 64) and the result
+// goes to Uc (LDS, row-major, ld kChunkLd) and back to S.
+template <typename T>
+__device__ void trsm_chunk(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb,
+                           const cx<T>* P, int ldp, cx<T>* Uc, int cb, int cw) {
+    constexpr int CPW = kChunk / kWaves;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t prow = lane < jb ? (size_t)rmap[j0 + lane] : 0;
+    cx<T> y[CPW];
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+        const int c = wave + kWaves * q;
+        y[q] = (lane < jb && c < cw) ? S[prow + (size_t)(cb + c) * ld] : czero<T>();
+    }
+    for (int k = 0; k < jb - 1; ++k) {
+        const cx<T> lk = (lane > k && lane < jb) ? P[lane + (size_t)k * ldp] : czero<T>();
+#pragma unroll
+        for (int q = 0; q < CPW; ++q) {
+            const cx<T> vk = cshfl(y[q], k);
+            cfms(y[q], lk, vk);
         }
-        __syncthreads();
-        for (int e = tid; e < np * cw; e += kThreads) {
-            const int q = e % np, c = e / np;
-            S[dst[q] + (size_t)(cb + c) * ld] = stash[e];
+    }
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+        const int c = wave + kWaves * q;
+        if (lane < jb && c < cw) {
+            Uc[lane * kChunkLd + c] = y[q];
+            S[prow + (size_t)(cb + c) * ld] = y[q];
         }
-        __syncthreads();
     }
 }
 
-// ---- trailing update of one column chunk (VALU) -------------------------------------
-// C = S[j0+jb : j0+m, cb : cb+cw] -= L21 U, L21 = P[jb:m, 0:jb] (LDS, ld ldp),
-// U = jb x cw (LDS, ld jb).  Thread tile: 4 rows x 2 columns.
+// ---- trailing update of one chunk on MFMA -------------------------------------------------
+// C = logical rows [j0+jb, n) x columns [cb, cb+cw) of S (physical rows via rmap)
+//   -= L21 U12,  L21 = P[jb:m, 0:jb] (LDS), U12 = Uc (jb x cw, LDS, row-major).
+// Computed transposed (C^T -= U^T L21^T) so that a lane's accumulators are 16
+// consecutive rows of one column: wave tile 32 rows x 32 columns (2 x 2 MFMA
+// tiles, real and imaginary accumulators, 4 real MFMAs per complex product).
 template <typename T>
-__device__ void trailing_update(cx<T>* S, int ld, const cx<T>* P, int ldp, int m, int jb,
-                                const cx<T>* U, int j0, int cb, int cw) {
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    for (int r0 = jb; r0 < m; r0 += 64) {
-        cx<T> acc[4][2];
+__device__ void trailing_mfma(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb, int m,
+                              const cx<T>* P, int ldp, const cx<T>* Uc, int cb, int cw) {
+    using M = Mfma<T>;
+    using acc_t = typename M::acc;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int m2 = m - jb;
+    for (int r0 = wave * 32; r0 < m2; r0 += kWaves * 32) {
+        acc_t cre[2][2], cim[2][2];
+        size_t prow[2];
+        bool rok[2];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int i = r0 + tx + 16 * a, c = ty + 16 * b;
-                acc[a][b] = (i < m && c < cw) ? S[(size_t)(j0 + i) + (size_t)(cb + c) * ld]
-                                              : czero<T>();
-            }
-        for (int k = 0; k < jb; ++k) {
-            cx<T> l[4], u[2];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int i = min(r0 + tx + 16 * a, m - 1);
-                l[a] = P[i + (size_t)k * ldp];
-            }
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int c = min(ty + 16 * b, cw - 1);
-                u[b] = U[k + c * jb];
-            }
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) cfms(acc[a][b], l[a], u[b]);
+        for (int a = 0; a < 2; ++a) {
+            const int row = r0 + 16 * a + li;
+            rok[a] = row < m2;
+            prow[a] = rok[a] ? (size_t)rmap[j0 + jb + row] : 0;
         }
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int i = r0 + tx + 16 * a, c = ty + 16 * b;
-                if (i < m && c < cw) S[(size_t)(j0 + i) + (size_t)(cb + c) * ld] = acc[a][b];
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = 16 * b + 4 * lg + r;
+                    cx<T> v = czero<T>();
+                    if (rok[a] && col < cw) v = S[prow[a] + (size_t)(cb + col) * ld];
+                    cre[a][b][r] = v.re;
+                    cim[a][b][r] = v.im;
+                }
+#pragma unroll 2
+        for (int k0 = 0; k0 < jb; k0 += 4) {
+            const int k = k0 + lg;
+            const bool kok = k < jb;
+            cx<T> lv[2], uv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int row = min(r0 + 16 * a + li, m2 - 1);
+                lv[a] = kok ? P[(jb + row) + (size_t)k * ldp] : czero<T>();
             }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) uv[b] = kok ? Uc[k * kChunkLd + 16 * b + li] : czero<T>();
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    cre[a][b] = M::run(-uv[b].re, lv[a].re, cre[a][b]);
+                    cre[a][b] = M::run(uv[b].im, lv[a].im, cre[a][b]);
+                    cim[a][b] = M::run(-uv[b].re, lv[a].im, cim[a][b]);
+                    cim[a][b] = M::run(-uv[b].im, lv[a].re, cim[a][b]);
+                }
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = 16 * b + 4 * lg + r;
+                    if (rok[a] && col < cw)
+                        S[prow[a] + (size_t)(cb + col) * ld] = cmk<T>(cre[a][b][r], cim[a][b][r]);
+                }
+    }
+}
+
+// ---- trailing update of one chunk on FP64 FMA (VALU) ----------------------------------------
+// Same wave tile and lane ownership as trailing_mfma (lane: rows r0+16a+li,
+// columns 16b+4lg+r); on MI355X FP64 FMA and FP64 MFMA share one peak rate.
+template <typename T>
+__device__ void trailing_valu(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb, int m,
+                              const cx<T>* P, int ldp, const cx<T>* Uc, int cb, int cw) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int m2 = m - jb;
+    for (int r0 = wave * 32; r0 < m2; r0 += kWaves * 32) {
+        cx<T> c[2][2][4];
+        size_t prow[2];
+        bool rok[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int row = r0 + 16 * a + li;
+            rok[a] = row < m2;
+            prow[a] = rok[a] ? (size_t)rmap[j0 + jb + row] : 0;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = 16 * b + 4 * lg + r;
+                    c[a][b][r] = (rok[a] && col < cw) ? S[prow[a] + (size_t)(cb + col) * ld]
+                                                      : czero<T>();
+                }
+        for (int k = 0; k < jb; ++k) {
+            cx<T> lv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) lv[a] = P[(jb + min(r0 + 16 * a + li, m2 - 1)) + (size_t)k * ldp];
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const cx<T> u = Uc[k * kChunkLd + 16 * b + 4 * lg + r];
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) cfms(c[a][b][r], lv[a], u);
+                }
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = 16 * b + 4 * lg + r;
+                    if (rok[a] && col < cw) S[prow[a] + (size_t)(cb + col) * ld] = c[a][b][r];
+                }
     }
 }
 
 // ---- LDS carve-up ---------------------------------------------------------------------
 struct Lds {
-    int* ipiv;
-    int* gperm;
-    int* pperm;
-    int* pdst;
-    int* psrc;
-    int* sc;
-    double* red;
+    int* ipiv;   // pivot row (0-based, logical) per column
+    int* rmap;   // logical row -> physical row (the composite row permutation)
+    int* sc;     // scalars: [0] pivot slot, [1] first zero pivot (1-based) or 0
+    double* red; // reduction scratch
     char* main;
 };
 
@@ -310,12 +423,8 @@ __device__ __forceinline__ Lds carve(char* smem, int n) {
     Lds L;
     int off = 0;
     L.ipiv = reinterpret_cast<int*>(smem);
-    L.gperm = L.ipiv + n;
-    L.pperm = L.gperm + n;
-    off += align16(3LL * n * 4);
-    L.pdst = reinterpret_cast<int*>(smem + off);
-    L.psrc = L.pdst + 128;
-    off += align16(4 * 64 * 4);
+    L.rmap = L.ipiv + n;
+    off += align16(2LL * n * 4);
     L.sc = reinterpret_cast<int*>(smem + off);
     off += align16(16 * 4);
     L.red = reinterpret_cast<double*>(smem + off);
@@ -324,255 +433,217 @@ __device__ __forceinline__ Lds carve(char* smem, int n) {
     return L;
 }
 
-// ---- blocked factorisation in a global slot ----------------------------------------------
+// ---- blocked right-looking factorisation in a global slot --------------------------------
+// Rows are never moved in S: a pivot swaps two entries of rmap and the panel
+// rows in LDS, so no row-interchange pass touches global memory.  Row r of
+// the LAPACK-layout factors is physical row rmap[r] of S.
 template <typename T, int MAXR>
-__device__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* S, const Layout& lay,
-                               const Lds& L) {
-    const int n = s.n, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+__device__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* __restrict__ S,
+                               const Layout& lay, const Lds& L) {
+    const int n = s.n, tid = threadIdx.x;
     const size_t nn = (size_t)n * n;
+#pragma unroll 4
     for (size_t e = tid; e < nn; e += kThreads) S[e] = to_t<T>(assemble_at(s, e, w, w2));
-    for (int r = tid; r < n; r += kThreads) L.gperm[r] = r;
+    for (int r = tid; r < n; r += kThreads) L.rmap[r] = r;
     if (tid == 0) L.sc[1] = 0;
     __syncthreads();
 
-    const int jbmax = lay.jb, CW = lay.cw;
+    const int jbmax = lay.jb;
     cx<T>* P = reinterpret_cast<cx<T>*>(L.main);
-    cx<T>* Linv = reinterpret_cast<cx<T>*>(L.main + align16((long long)n * jbmax * sizeof(cx<T>)));
-    cx<T>* Wk = reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(Linv) +
-                                        align16((long long)jbmax * jbmax * sizeof(cx<T>)));
+    cx<T>* Uc = reinterpret_cast<cx<T>*>(L.main + align16((long long)n * jbmax * sizeof(cx<T>)));
 
     for (int j0 = 0; j0 < n; j0 += jbmax) {
         const int jb = min(jbmax, n - j0), m = n - j0;
         for (int e = tid; e < m * jb; e += kThreads) {
             const int i = e % m, jj = e / m;
-            P[e] = S[(size_t)(j0 + i) + (size_t)(j0 + jj) * n];
+            P[e] = S[(size_t)L.rmap[j0 + i] + (size_t)(j0 + jj) * n];
         }
-        for (int r = tid; r < m; r += kThreads) L.pperm[r] = r;
         __syncthreads();
-        panel_factor<T, MAXR>(P, m, m, jb, j0, L.ipiv, L.gperm, L.pperm, L.sc);
+        panel_factor<T, MAXR>(P, m, m, jb, j0, L.ipiv, L.rmap, L.sc);
         for (int e = tid; e < m * jb; e += kThreads) {
             const int i = e % m, jj = e / m;
-            S[(size_t)(j0 + i) + (size_t)(j0 + jj) * n] = P[e];
+            S[(size_t)L.rmap[j0 + i] + (size_t)(j0 + jj) * n] = P[e];
         }
-        // moved rows of this panel: candidates are the panel rows and the pivot rows
-        if (wave == 0) {
-            int cnt = 0;
-            for (int base = 0; base < 2 * jb; base += 64) {
-                const int q = base + lane;
-                bool keep = false;
-                int r = 0;
-                if (q < 2 * jb) {
-                    r = q < jb ? q : L.ipiv[j0 + q - jb] - j0;
-                    keep = L.pperm[r] != r;
-                    for (int q2 = 0; keep && q2 < q; ++q2) {
-                        const int r2 = q2 < jb ? q2 : L.ipiv[j0 + q2 - jb] - j0;
-                        if (r2 == r) keep = false;
-                    }
-                }
-                const unsigned long long mask = __ballot(keep);
-                const int pos = __popcll(mask & ((1ull << lane) - 1ull));
-                if (keep) {
-                    L.pdst[cnt + pos] = j0 + r;
-                    L.psrc[cnt + pos] = j0 + L.pperm[r];
-                }
-                cnt += __popcll(mask);
-            }
-            if (lane == 0) L.sc[2] = cnt;
-        }
-        // Linv = inverse of the unit-lower L11 (jb <= 64): one column per wave pass
-        for (int j = wave; j < jb; j += kWaves) {
-            cx<T> x = lane == j ? cmk<T>(T(1), T(0)) : czero<T>();
-            for (int k = j; k < jb - 1; ++k) {
-                const cx<T> xk = cshfl(x, k);
-                if (lane > k && lane < jb) cfms(x, P[lane + (size_t)k * m], xk);
-            }
-            if (lane < jb) Linv[lane + j * jb] = x;
+        for (int cb = j0 + jb; cb < n; cb += kChunk) {
+            const int cw = min(kChunk, n - cb);
+            trsm_chunk(S, n, L.rmap, j0, jb, P, m, Uc, cb, cw);
+            __syncthreads();
+            if constexpr (sizeof(T) == 8)
+                trailing_valu(S, n, L.rmap, j0, jb, m, P, m, Uc, cb, cw);
+            else
+                trailing_mfma(S, n, L.rmap, j0, jb, m, P, m, Uc, cb, cw);
+            __syncthreads();
         }
         __syncthreads();
-        const int np = L.sc[2];
-        apply_moves(S, n, 0, j0, L.pdst, L.psrc, np, Wk, CW);
-        for (int cb = j0 + jb; cb < n; cb += CW) {
-            const int cw = min(CW, n - cb);
-            apply_moves(S, n, cb, cb + cw, L.pdst, L.psrc, np, Wk, CW);
-            for (int e = tid; e < jb * cw; e += kThreads) {
-                const int k = e % jb, c = e / jb;
-                Wk[e] = S[(size_t)(j0 + k) + (size_t)(cb + c) * n];
-            }
-            __syncthreads();
-            cx<T>* U = Wk + jb * CW;
-            for (int e = tid; e < jb * cw; e += kThreads) {
-                const int i = e % jb, c = e / jb;
-                cx<T> acc = Wk[i + c * jb];
-                for (int k = 0; k < i; ++k) cfma(acc, Linv[i + k * jb], Wk[k + c * jb]);
-                U[e] = acc;
-                S[(size_t)(j0 + i) + (size_t)(cb + c) * n] = acc;
-            }
-            __syncthreads();
-            trailing_update(S, n, P, m, m, jb, U, j0, cb, cw);
-            __syncthreads();
-        }
     }
 }
 
 // ---- triangular solves on LDS vectors w (n x gc, ld n) -----------------------------------
-// L unit lower (in LU), forward.
+// LU: factors with logical row r at physical row rmap[r] (rmap == nullptr:
+// identity).  Dg: LDS staging for one kSolveBlock^2 diagonal block, or nullptr
+// when LU itself is in LDS.  Off-diagonal updates issue 8 loads per round.
 template <typename T>
-__device__ void lower_unit_solve(const cx<T>* LU, int ld, int n, cx<T>* w, int gc) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int k0 = 0; k0 < n; k0 += 64) {
-        const int kb = min(64, n - k0);
+struct TriCtx {
+    const cx<T>* LU;
+    int ld;
+    int n;
+    const int* rmap;
+    cx<T>* Dg;
+    __device__ __forceinline__ size_t row(int i) const { return rmap ? (size_t)rmap[i] : (size_t)i; }
+    __device__ __forceinline__ cx<T> at(int i, int j) const { return LU[row(i) + (size_t)j * ld]; }
+};
+
+// Copy the diagonal block [k0, k0+kb)^2 into Dg (when staging); returns the
+// accessor base and leading dimension for that block.
+template <typename T>
+__device__ __forceinline__ const cx<T>* stage_diag(const TriCtx<T>& t, int k0, int kb, int* ldd,
+                                                   bool* mapped) {
+    if (t.Dg) {
+        for (int e = threadIdx.x; e < kb * kb; e += kThreads) {
+            const int i = e % kb, j = e / kb;
+            t.Dg[i + j * kSolveBlock] = t.at(k0 + i, k0 + j);
+        }
+        __syncthreads();
+        *ldd = kSolveBlock;
+        *mapped = false;
+        return t.Dg;
+    }
+    *ldd = t.ld;
+    *mapped = false;
+    return t.LU + k0 + (size_t)k0 * t.ld;
+}
+
+template <typename T, bool CONJ, bool TRANS>
+__device__ __forceinline__ void rect_update(const TriCtx<T>& t, int i, int k0, int kb, cx<T>* w,
+                                            int gc) {
+    // w[i] -= sum_{k<kb} op(LU)(i, k0+k) w[k0+k];  op = (TRANS ? element (k0+k, i) : (i, k0+k)),
+    // conjugated when CONJ
+    cx<T> acc[kMaxGroup];
+#pragma unroll
+    for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
+    const size_t pi = t.row(i);
+    for (int kk = 0; kk < kb; kk += 8) {
+        cx<T> l[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = k0 + kk + u;
+            l[u] = czero<T>();
+            if (kk + u < kb) l[u] = TRANS ? t.LU[t.row(k) + (size_t)i * t.ld] : t.LU[pi + (size_t)k * t.ld];
+            if (CONJ) l[u] = cconj(l[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (kk + u < kb)
+#pragma unroll
+                for (int g = 0; g < kMaxGroup; ++g)
+                    if (g < gc) cfma(acc[g], l[u], w[k0 + kk + u + (size_t)g * t.n]);
+    }
+#pragma unroll
+    for (int g = 0; g < kMaxGroup; ++g)
+        if (g < gc) {
+            cx<T> a = w[i + (size_t)g * t.n];
+            a.re -= acc[g].re;
+            a.im -= acc[g].im;
+            w[i + (size_t)g * t.n] = a;
+        }
+}
+
+// L unit lower, forward.
+template <typename T>
+__device__ void lower_unit_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, n = t.n;
+    for (int k0 = 0; k0 < n; k0 += kSolveBlock) {
+        const int kb = min(kSolveBlock, n - k0);
+        int ldd;
+        bool mp;
+        const cx<T>* Db = stage_diag(t, k0, kb, &ldd, &mp);
         for (int g = wave; g < gc; g += kWaves) {
             cx<T>* y = w + (size_t)g * n + k0;
             cx<T> v = lane < kb ? y[lane] : czero<T>();
-            const cx<T>* Lb = LU + k0 + (size_t)k0 * ld;
             for (int k = 0; k < kb - 1; ++k) {
                 const cx<T> vk = cshfl(v, k);
-                if (lane > k && lane < kb) cfms(v, Lb[lane + (size_t)k * ld], vk);
+                if (lane > k && lane < kb) cfms(v, Db[lane + (size_t)k * ldd], vk);
             }
             if (lane < kb) y[lane] = v;
         }
         __syncthreads();
-        for (int i = k0 + kb + tid; i < n; i += kThreads) {
-            cx<T> acc[kMaxGroup];
-#pragma unroll
-            for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
-            for (int k = 0; k < kb; ++k) {
-                const cx<T> l = LU[i + (size_t)(k0 + k) * ld];
-#pragma unroll
-                for (int g = 0; g < kMaxGroup; ++g)
-                    if (g < gc) cfma(acc[g], l, w[k0 + k + (size_t)g * n]);
-            }
-#pragma unroll
-            for (int g = 0; g < kMaxGroup; ++g)
-                if (g < gc) {
-                    cx<T> a = w[i + (size_t)g * n];
-                    a.re -= acc[g].re;
-                    a.im -= acc[g].im;
-                    w[i + (size_t)g * n] = a;
-                }
-        }
+        for (int i = k0 + kb + tid; i < n; i += kThreads) rect_update<T, false, false>(t, i, k0, kb, w, gc);
         __syncthreads();
     }
 }
 
 // U upper, backward.
 template <typename T>
-__device__ void upper_solve(const cx<T>* LU, int ld, int n, cx<T>* w, int gc) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int k1 = n; k1 > 0; k1 -= 64) {
-        const int k0 = max(0, k1 - 64), kb = k1 - k0;
+__device__ void upper_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, n = t.n;
+    const int nb = (n + kSolveBlock - 1) / kSolveBlock;
+    for (int b = nb - 1; b >= 0; --b) {
+        const int k0 = b * kSolveBlock, kb = min(kSolveBlock, n - k0);
+        int ldd;
+        bool mp;
+        const cx<T>* Db = stage_diag(t, k0, kb, &ldd, &mp);
         for (int g = wave; g < gc; g += kWaves) {
             cx<T>* y = w + (size_t)g * n + k0;
             cx<T> v = lane < kb ? y[lane] : czero<T>();
-            const cx<T>* Ub = LU + k0 + (size_t)k0 * ld;
             for (int k = kb - 1; k >= 0; --k) {
-                if (lane == k) v = cdiv(v, Ub[k + (size_t)k * ld]);
+                if (lane == k) v = cdiv(v, Db[k + (size_t)k * ldd]);
                 const cx<T> vk = cshfl(v, k);
-                if (lane < k) cfms(v, Ub[lane + (size_t)k * ld], vk);
+                if (lane < k) cfms(v, Db[lane + (size_t)k * ldd], vk);
             }
             if (lane < kb) y[lane] = v;
         }
         __syncthreads();
-        for (int i = tid; i < k0; i += kThreads) {
-            cx<T> acc[kMaxGroup];
-#pragma unroll
-            for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
-            for (int k = 0; k < kb; ++k) {
-                const cx<T> u = LU[i + (size_t)(k0 + k) * ld];
-#pragma unroll
-                for (int g = 0; g < kMaxGroup; ++g)
-                    if (g < gc) cfma(acc[g], u, w[k0 + k + (size_t)g * n]);
-            }
-#pragma unroll
-            for (int g = 0; g < kMaxGroup; ++g)
-                if (g < gc) {
-                    cx<T> a = w[i + (size_t)g * n];
-                    a.re -= acc[g].re;
-                    a.im -= acc[g].im;
-                    w[i + (size_t)g * n] = a;
-                }
-        }
+        for (int i = tid; i < k0; i += kThreads) rect_update<T, false, false>(t, i, k0, kb, w, gc);
         __syncthreads();
     }
 }
 
 // U^H y = b, forward (adjoint).
 template <typename T>
-__device__ void upper_h_solve(const cx<T>* LU, int ld, int n, cx<T>* w, int gc) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int k0 = 0; k0 < n; k0 += 64) {
-        const int kb = min(64, n - k0);
+__device__ void upper_h_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, n = t.n;
+    for (int k0 = 0; k0 < n; k0 += kSolveBlock) {
+        const int kb = min(kSolveBlock, n - k0);
+        int ldd;
+        bool mp;
+        const cx<T>* Db = stage_diag(t, k0, kb, &ldd, &mp);
         for (int g = wave; g < gc; g += kWaves) {
             cx<T>* y = w + (size_t)g * n + k0;
             cx<T> v = lane < kb ? y[lane] : czero<T>();
-            const cx<T>* Ub = LU + k0 + (size_t)k0 * ld;
             for (int k = 0; k < kb; ++k) {
-                if (lane == k) v = cdiv(v, cconj(Ub[k + (size_t)k * ld]));
+                if (lane == k) v = cdiv(v, cconj(Db[k + (size_t)k * ldd]));
                 const cx<T> vk = cshfl(v, k);
-                if (lane > k && lane < kb) cfms(v, cconj(Ub[k + (size_t)lane * ld]), vk);
+                if (lane > k && lane < kb) cfms(v, cconj(Db[k + (size_t)lane * ldd]), vk);
             }
             if (lane < kb) y[lane] = v;
         }
         __syncthreads();
-        for (int i = k0 + kb + tid; i < n; i += kThreads) {
-            cx<T> acc[kMaxGroup];
-#pragma unroll
-            for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
-            for (int k = 0; k < kb; ++k) {
-                const cx<T> u = cconj(LU[(k0 + k) + (size_t)i * ld]);
-#pragma unroll
-                for (int g = 0; g < kMaxGroup; ++g)
-                    if (g < gc) cfma(acc[g], u, w[k0 + k + (size_t)g * n]);
-            }
-#pragma unroll
-            for (int g = 0; g < kMaxGroup; ++g)
-                if (g < gc) {
-                    cx<T> a = w[i + (size_t)g * n];
-                    a.re -= acc[g].re;
-                    a.im -= acc[g].im;
-                    w[i + (size_t)g * n] = a;
-                }
-        }
+        for (int i = k0 + kb + tid; i < n; i += kThreads) rect_update<T, true, true>(t, i, k0, kb, w, gc);
         __syncthreads();
     }
 }
 
 // L^H v = y, backward, unit diagonal (adjoint).
 template <typename T>
-__device__ void lower_h_unit_solve(const cx<T>* LU, int ld, int n, cx<T>* w, int gc) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int k1 = n; k1 > 0; k1 -= 64) {
-        const int k0 = max(0, k1 - 64), kb = k1 - k0;
+__device__ void lower_h_unit_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, n = t.n;
+    const int nb = (n + kSolveBlock - 1) / kSolveBlock;
+    for (int b = nb - 1; b >= 0; --b) {
+        const int k0 = b * kSolveBlock, kb = min(kSolveBlock, n - k0);
+        int ldd;
+        bool mp;
+        const cx<T>* Db = stage_diag(t, k0, kb, &ldd, &mp);
         for (int g = wave; g < gc; g += kWaves) {
             cx<T>* y = w + (size_t)g * n + k0;
             cx<T> v = lane < kb ? y[lane] : czero<T>();
-            const cx<T>* Lb = LU + k0 + (size_t)k0 * ld;
             for (int k = kb - 1; k > 0; --k) {
                 const cx<T> vk = cshfl(v, k);
-                if (lane < k) cfms(v, cconj(Lb[k + (size_t)lane * ld]), vk);
+                if (lane < k) cfms(v, cconj(Db[k + (size_t)lane * ldd]), vk);
             }
             if (lane < kb) y[lane] = v;
         }
         __syncthreads();
-        for (int i = tid; i < k0; i += kThreads) {
-            cx<T> acc[kMaxGroup];
-#pragma unroll
-            for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
-            for (int k = 0; k < kb; ++k) {
-                const cx<T> l = cconj(LU[(k0 + k) + (size_t)i * ld]);
-#pragma unroll
-                for (int g = 0; g < kMaxGroup; ++g)
-                    if (g < gc) cfma(acc[g], l, w[k0 + k + (size_t)g * n]);
-            }
-#pragma unroll
-            for (int g = 0; g < kMaxGroup; ++g)
-                if (g < gc) {
-                    cx<T> a = w[i + (size_t)g * n];
-                    a.re -= acc[g].re;
-                    a.im -= acc[g].im;
-                    w[i + (size_t)g * n] = a;
-                }
-        }
+        for (int i = tid; i < k0; i += kThreads) rect_update<T, true, true>(t, i, k0, kb, w, gc);
         __syncthreads();
     }
 }
@@ -638,9 +709,9 @@ __device__ void residual(const SysDesc& s, double w, double w2, const double2* b
                          const z64* bl, const z64* xs, z64* rs, int gc, bool need_anorm,
                          double* red, double (&out)[kRedSlots]) {
     const int n = s.n, tid = threadIdx.x;
-    double v[kRedSlots];
-#pragma unroll
-    for (int q = 0; q < kRedSlots; ++q) v[q] = 0.0;
+    // pass 1: r = b - op(A) x (and the row sums of |op(A)|), 4 columns of the
+    // operators in flight per round
+    double amax = 0.0;
     for (int i = tid; i < n; i += kThreads) {
         z64 acc[kMaxGroup];
 #pragma unroll
@@ -648,24 +719,59 @@ __device__ void residual(const SysDesc& s, double w, double w2, const double2* b
             acc[g] = g < gc ? (bg ? ld_z(bg + i + (size_t)g * ldb) : bl[i + (size_t)g * n])
                             : czero<double>();
         double asum = 0.0;
-        for (int j = 0; j < n; ++j) {
-            z64 a = ADJ ? cconj(assemble_at(s, (size_t)j + (size_t)i * n, w, w2))
-                        : assemble_at(s, (size_t)i + (size_t)j * n, w, w2);
-            if (need_anorm) asum += zabs(a);
+        for (int j0 = 0; j0 < n; j0 += 4) {
+            z64 a[4];
+            if (s.real_ops) {
+                const double* __restrict__ Kp = s.K;
+                const double* __restrict__ Mp = s.M;
+                const double* __restrict__ Dp = s.D;
 #pragma unroll
-            for (int g = 0; g < kMaxGroup; ++g)
-                if (g < gc) cfms(acc[g], a, xs[j + (size_t)g * n]);
+                for (int u = 0; u < 4; ++u) {
+                    const int j = min(j0 + u, n - 1);
+                    const size_t e = ADJ ? (size_t)j + (size_t)i * n : (size_t)i + (size_t)j * n;
+                    const double k = Kp[e], mm = Mp[e];
+                    const double d = s.has_D ? Dp[e] : 0.0;
+                    a[u] = cmk<double>(k - w2 * mm, (ADJ ? -1.0 : 1.0) * (s.eta * k + w * d));
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = min(j0 + u, n - 1);
+                    a[u] = ADJ ? cconj(assemble_at(s, (size_t)j + (size_t)i * n, w, w2))
+                               : assemble_at(s, (size_t)i + (size_t)j * n, w, w2);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (j0 + u < n) {
+                    if (need_anorm) asum += zabs(a[u]);
+#pragma unroll
+                    for (int g = 0; g < kMaxGroup; ++g)
+                        if (g < gc) cfms(acc[g], a[u], xs[j0 + u + (size_t)g * n]);
+                }
+            }
         }
-        v[0] = nmax(v[0], asum);
+        amax = nmax(amax, asum);
+#pragma unroll
+        for (int g = 0; g < kMaxGroup; ++g)
+            if (g < gc) rs[i + (size_t)g * n] = acc[g];
+    }
+    __syncthreads();
+    // pass 2: the norms of the backward error
+    double v[kRedSlots];
+#pragma unroll
+    for (int q = 0; q < kRedSlots; ++q) v[q] = 0.0;
+    v[0] = amax;
+    for (int i = tid; i < n; i += kThreads) {
 #pragma unroll
         for (int g = 0; g < kMaxGroup; ++g)
             if (g < gc) {
-                rs[i + (size_t)g * n] = acc[g];
+                const z64 r = rs[i + (size_t)g * n];
                 const z64 b = bg ? ld_z(bg + i + (size_t)g * ldb) : bl[i + (size_t)g * n];
-                v[1 + 4 * g] = nmax(v[1 + 4 * g], zabs(acc[g]));
+                v[1 + 4 * g] = nmax(v[1 + 4 * g], zabs(r));
                 v[2 + 4 * g] = nmax(v[2 + 4 * g], zabs(xs[i + (size_t)g * n]));
                 v[3 + 4 * g] = nmax(v[3 + 4 * g], zabs(b));
-                v[4 + 4 * g] = nmax(v[4 + 4 * g], cabs1(acc[g]));
+                v[4 + 4 * g] = nmax(v[4 + 4 * g], cabs1(r));
             }
     }
     block_max(v, red);
@@ -685,26 +791,26 @@ __device__ __forceinline__ double backward_error(double rmax, double anorm, doub
 // On exit xs holds the solution (FP64).  Returns refinement steps; conv[] = 1
 // for columns meeting tol; berr_out[] their backward errors (when computed).
 template <typename T, bool ADJ>
-__device__ int solve_group(const SysDesc& s, double w, double w2, const cx<T>* LU, int ld,
-                           const int* gperm, cx<T>* wv, z64* xs, z64* rs, int gc,
+__device__ int solve_group(const SysDesc& s, double w, double w2, const TriCtx<T>& tri,
+                           const int* perm, cx<T>* wv, z64* xs, z64* rs, int gc,
                            const double2* bg, int ldb, const z64* bl, bool refine, bool want_berr,
                            double tol, int max_iter, double* anorm,
                            double (&berr_out)[kMaxGroup], bool* allconv, double* red) {
     const int n = s.n, tid = threadIdx.x;
     auto solve_t = [&]() {
         if (!ADJ) {
-            lower_unit_solve(LU, ld, n, wv, gc);
-            upper_solve(LU, ld, n, wv, gc);
+            lower_unit_solve(tri, wv, gc);
+            upper_solve(tri, wv, gc);
         } else {
-            upper_h_solve(LU, ld, n, wv, gc);
-            lower_h_unit_solve(LU, ld, n, wv, gc);
+            upper_h_solve(tri, wv, gc);
+            lower_h_unit_solve(tri, wv, gc);
         }
     };
-    // x = op(A)^{-1} b in T; ADJ leaves the result un-permuted (x[gperm[r]] = wv[r])
+    // x = op(A)^{-1} b in T; ADJ leaves the result un-permuted (x[perm[r]] = wv[r])
     solve_t();
     for (int e = tid; e < n * gc; e += kThreads) {
         const int r = e % n, g = e / n;
-        const int dst = ADJ ? gperm[r] : r;
+        const int dst = ADJ ? perm[r] : r;
         xs[dst + (size_t)g * n] = to_z(wv[e]);
     }
     __syncthreads();
@@ -742,7 +848,7 @@ __device__ int solve_group(const SysDesc& s, double w, double w2, const cx<T>* L
         for (int e = tid; e < n * gc; e += kThreads) {
             const int r = e % n, g = e / n;
             const double sg = scl[g];
-            const z64 rv = rs[(ADJ ? r : gperm[r]) + (size_t)g * n];
+            const z64 rv = rs[(ADJ ? r : perm[r]) + (size_t)g * n];
             wv[e] = to_t<T>(cmk<double>(rv.re / sg, rv.im / sg));
         }
         __syncthreads();
@@ -750,7 +856,7 @@ __device__ int solve_group(const SysDesc& s, double w, double w2, const cx<T>* L
         for (int e = tid; e < n * gc; e += kThreads) {
             const int r = e % n, g = e / n;
             const double sg = scl[g];
-            const int dst = ADJ ? gperm[r] : r;
+            const int dst = ADJ ? perm[r] : r;
             z64 x = xs[dst + (size_t)g * n];
             const z64 d = to_z(wv[e]);
             x.re = fma(sg, d.re, x.re);
@@ -770,7 +876,7 @@ __device__ int solve_group(const SysDesc& s, double w, double w2, const cx<T>* L
 // = {J_w, dJ/deta_w, dJ/ds_0,w, ...} and info[fidx] (0, or -1 when a refined
 // solve missed tol).
 template <typename T>
-__device__ void adjoint_point(const SysDesc& s, int fidx, double w, double w2, const cx<T>* LU,
+__device__ void adjoint_point(const SysDesc& s, int fidx, double w, double w2, const TriCtx<T>& tri,
                               const Lds& L, cx<T>* wv, z64* xs, z64* rs, z64* xk, z64* bv,
                               bool refine, double tol, int max_iter) {
     const int n = s.n, tid = threadIdx.x, p = s.p;
@@ -780,11 +886,11 @@ __device__ void adjoint_point(const SysDesc& s, int fidx, double w, double w2, c
     bool ok = true;
     for (int r = 0; r < s.nrhs; ++r) {
         const double2* Fr = s.F + (size_t)r * s.ldf;
-        for (int i = tid; i < n; i += kThreads) wv[i] = to_t<T>(ld_z(Fr + L.gperm[i]));
+        for (int i = tid; i < n; i += kThreads) wv[i] = to_t<T>(ld_z(Fr + L.rmap[i]));
         __syncthreads();
         double be[kMaxGroup];
         bool conv;
-        solve_group<T, false>(s, w, w2, LU, n, L.gperm, wv, xs, rs, 1, Fr, s.ldf, nullptr, refine,
+        solve_group<T, false>(s, w, w2, tri, L.rmap, wv, xs, rs, 1, Fr, s.ldf, nullptr, refine,
                               refine, tol, max_iter, &anorm_f, be, &conv, L.red);
         if (!conv) ok = false;
         // g = c^H x
@@ -809,7 +915,7 @@ __device__ void adjoint_point(const SysDesc& s, int fidx, double w, double w2, c
             wv[i] = to_t<T>(b);
         }
         __syncthreads();
-        solve_group<T, true>(s, w, w2, LU, n, L.gperm, wv, xs, rs, 1, nullptr, n, bv, refine,
+        solve_group<T, true>(s, w, w2, tri, L.rmap, wv, xs, rs, 1, nullptr, n, bv, refine,
                              refine, tol, max_iter, &anorm_a, be, &conv, L.red);
         if (!conv) ok = false;
         // mu^H (K x)
@@ -876,16 +982,20 @@ __device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, cha
         cx<T>* A = reinterpret_cast<cx<T>*>(L.main);
         const size_t nn = (size_t)n * n;
         for (size_t e = tid; e < nn; e += kThreads) A[e] = to_t<T>(assemble_at(s, e, w, w2));
-        for (int r = tid; r < n; r += kThreads) L.gperm[r] = r;
+        for (int r = tid; r < n; r += kThreads) L.rmap[r] = r;
         if (tid == 0) L.sc[1] = 0;
         __syncthreads();
-        panel_factor<T, MAXR>(A, n, n, n, 0, L.ipiv, L.gperm, nullptr, L.sc);
+        panel_factor<T, MAXR>(A, n, n, n, 0, L.ipiv, L.rmap, L.sc);
         LU = A;
         solvep = L.main + align16((long long)nn * sizeof(cx<T>));
     } else {
+        if (kp.scratch == nullptr || kp.slot_elems < (size_t)n * n) {
+            // the launch did not provide a factor slot: report, touch nothing
+            if (tid == 0) s.info[fidx] = -2;
+            __syncthreads();
+            return;
+        }
         cx<T>* S = reinterpret_cast<cx<T>*>(kp.scratch) + (size_t)blockIdx.x * kp.slot_elems;
-        if (kp.flags & kFactorOut)
-            S = reinterpret_cast<cx<T>*>(s.factor_out) + (size_t)fidx * n * n;
         factor_blocked<T, MAXR>(s, w, w2, S, lay, L);
         LU = S;
         solvep = L.main;
@@ -893,9 +1003,12 @@ __device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, cha
     const int info = L.sc[1];
 
     if (kp.flags & kFactorOut) {
-        if (lay.small) {
-            cx<T>* out = reinterpret_cast<cx<T>*>(s.factor_out) + (size_t)fidx * n * n;
-            for (size_t e = tid; e < (size_t)n * n; e += kThreads) out[e] = LU[e];
+        // LAPACK layout: row r of the factors is physical row rmap[r] (blocked path)
+        cx<T>* out = reinterpret_cast<cx<T>*>(s.factor_out) + (size_t)fidx * n * n;
+        for (size_t e = tid; e < (size_t)n * n; e += kThreads) {
+            const int r = (int)(e % n);
+            const size_t c = e / n;
+            out[e] = lay.small ? LU[e] : LU[(size_t)L.rmap[r] + c * n];
         }
         for (int r = tid; r < n; r += kThreads) s.ipiv_out[(size_t)fidx * n + r] = L.ipiv[r] + 1;
         if (tid == 0) s.info[fidx] = info;
@@ -906,12 +1019,15 @@ __device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, cha
     double2* X = s.X + (size_t)fidx * s.ldx * nrhs;
     if (info != 0) {
         const double qnan = __builtin_nan("");
-        for (int e = tid; e < n * nrhs; e += kThreads) {
-            const int r = e % n, g = e / n;
-            X[r + (size_t)g * s.ldx] = make_double2(qnan, qnan);
-        }
+        if (X)
+            for (int e = tid; e < n * nrhs; e += kThreads) {
+                const int r = e % n, g = e / n;
+                X[r + (size_t)g * s.ldx] = make_double2(qnan, qnan);
+            }
         if (s.berr)
             for (int g = tid; g < nrhs; g += kThreads) s.berr[(size_t)fidx * nrhs + g] = __builtin_inf();
+        if (s.adj_out && tid == 0)
+            for (int q = 0; q < s.p + 2; ++q) s.adj_out[(size_t)fidx * (s.p + 2) + q] = 0.0;
         if (tid == 0) {
             s.info[fidx] = info;
             if (s.iters) s.iters[fidx] = 0;
@@ -921,13 +1037,21 @@ __device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, cha
     }
 
     const int G = lay.G;
+    const int nvec = (kp.flags & kAdjoint) ? 4 : 2;
     cx<T>* wv = reinterpret_cast<cx<T>*>(solvep);
     z64* xs = reinterpret_cast<z64*>(solvep + align16((long long)n * G * sizeof(cx<T>)));
     z64* rs = reinterpret_cast<z64*>(reinterpret_cast<char*>(xs) + align16((long long)n * G * 16));
+    char* after = reinterpret_cast<char*>(xs) + nvec * align16((long long)n * G * 16);
+    TriCtx<T> tri;
+    tri.LU = LU;
+    tri.ld = n;
+    tri.n = n;
+    tri.rmap = lay.small ? nullptr : L.rmap;
+    tri.Dg = lay.small ? nullptr : reinterpret_cast<cx<T>*>(after);
     if (kp.flags & kAdjoint) {
         z64* xk = reinterpret_cast<z64*>(reinterpret_cast<char*>(rs) + align16((long long)n * G * 16));
         z64* bv = reinterpret_cast<z64*>(reinterpret_cast<char*>(xk) + align16((long long)n * G * 16));
-        adjoint_point<T>(s, fidx, w, w2, LU, L, wv, xs, rs, xk, bv, refine, kp.tol, kp.max_iter);
+        adjoint_point<T>(s, fidx, w, w2, tri, L, wv, xs, rs, xk, bv, refine, kp.tol, kp.max_iter);
         return;
     }
     double anorm = -1.0;
@@ -938,12 +1062,12 @@ __device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, cha
         const double2* Fg = s.F + (size_t)g0 * s.ldf;
         for (int e = tid; e < n * gc; e += kThreads) {
             const int r = e % n, g = e / n;
-            wv[e] = to_t<T>(ld_z(Fg + L.gperm[r] + (size_t)g * s.ldf));
+            wv[e] = to_t<T>(ld_z(Fg + L.rmap[r] + (size_t)g * s.ldf));
         }
         __syncthreads();
         double be[kMaxGroup];
         bool conv;
-        const int it = solve_group<T, false>(s, w, w2, LU, n, L.gperm, wv, xs, rs, gc, Fg, s.ldf,
+        const int it = solve_group<T, false>(s, w, w2, tri, L.rmap, wv, xs, rs, gc, Fg, s.ldf,
                                              nullptr, refine, want_berr, kp.tol, kp.max_iter,
                                              &anorm, be, &conv, L.red);
         if (refine && !conv) ok = false;
@@ -965,25 +1089,43 @@ __device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, cha
     }
 }
 
+// ---- descriptor in scalar registers --------------------------------------------------------
+// Every dword passes through readfirstlane, so the compiler knows the fields
+// are wavefront-uniform and keeps them (and the addresses built from them) in
+// SGPRs instead of spending vector registers on them.
+__device__ __forceinline__ SysDesc load_uniform(const SysDesc* p) {
+    static_assert(sizeof(SysDesc) % 4 == 0, "descriptor size");
+    SysDesc d;
+    const int* src = reinterpret_cast<const int*>(p);
+    int* dst = reinterpret_cast<int*>(&d);
+#pragma unroll
+    for (int q = 0; q < (int)(sizeof(SysDesc) / 4); ++q) dst[q] = __builtin_amdgcn_readfirstlane(src[q]);
+    return d;
+}
+
 // ---- the persistent kernel -------------------------------------------------------------
+// Two workgroups (8 wavefronts) per CU: the register file is split two ways,
+// so each wavefront gets at most 256 registers.
 template <typename T, int MAXR>
-__global__ void __launch_bounds__(kThreads) sweep_kernel(KParams kp) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+sweep_kernel(KParams kp) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ int s_item;
     const int tid = threadIdx.x;
     for (;;) {
         if (tid == 0) s_item = atomicAdd(kp.counter, 1);
         __syncthreads();
-        const int item = s_item;
+        // uniform across the workgroup: keeps the descriptor in scalar registers
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (item >= kp.nitems) break;
         int sysi = 0, fidx = item;
         if (kp.items) {
             const int2 it = kp.items[item];
-            sysi = it.x;
-            fidx = it.y;
+            sysi = __builtin_amdgcn_readfirstlane(it.x);
+            fidx = __builtin_amdgcn_readfirstlane(it.y);
         }
-        const SysDesc s = kp.sys ? kp.sys[sysi] : kp.one;
+        const SysDesc s = kp.sys ? load_uniform(kp.sys + sysi) : load_uniform(&kp.one);
         process_point<T, MAXR>(s, fidx, kp, smem);
         __syncthreads();
     }
