@@ -126,6 +126,9 @@ def main():
     ap.add_argument("--no-vendor", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="both", choices=["both", "refined", "plain"])
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run 3 device-to-device copies of MIB MiB (a kernel of known "
+                         "traffic, to scale rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -166,6 +169,15 @@ def main():
         fe = np.abs(X - Xc).max(axis=(1, 2)) / np.abs(Xc).max(axis=(1, 2))
         return {"max_backward_error": float(be.max()), "points_failed": int((info != 0).sum()),
                 "max_rel_error_vs_closed_form": float(fe.max())}
+
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration * 1024 * 1024
+        a = torch.ones(nb // 4, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     res = {}
     for mode, tol in (("refined", TOL_REFINED), ("plain", TOL_PLAIN)):

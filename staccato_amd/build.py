@@ -11,6 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc", "stc_api.hip")
 OUT_DIR = os.path.join(HERE, "lib")
 OUT = os.path.join(OUT_DIR, "libstaccato_sweep.so")
+HOST_SRC = os.path.join(HERE, "csrc", "host", "KrylovROMSweep.cpp")
+HOST_OUT = os.path.join(OUT_DIR, "libstaccato_host.so")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -35,8 +37,16 @@ def build_library(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, OUT)
+    # C++ mirror of the reference's sweep entry, over the C ABI
+    hcmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", HOST_OUT + ".tmp", HOST_SRC,
+            "-L" + OUT_DIR, "-lstaccato_sweep", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(hcmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + r.stdout + r.stderr)
+    os.replace(HOST_OUT + ".tmp", HOST_OUT)
     if verbose:
         print(" ".join(cmd))
+        print(" ".join(hcmd))
     return OUT
 
 

@@ -102,11 +102,10 @@ constexpr int kBudgetSmall = 80 * 1024;      // two workgroups per CU
 constexpr int kBudgetLarge = 160 * 1024 - 256;  // one workgroup per CU
 
 // Pick the LDS budget of a launch and fill each system's layout.
-int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int* lds_out) {
+int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int jb_cap, int* lds_out) {
     int budget = kBudgetSmall;
     if (const char* e = getenv("STC_LDS_BUDGET")) budget = std::max(16384, std::min(atoi(e), kBudgetLarge));
-    int jb_cap = 64;
-    if (const char* e = getenv("STC_JB_MAX")) jb_cap = std::max(1, std::min(atoi(e), 64));
+    if (const char* e = getenv("STC_JB_MAX")) jb_cap = std::max(4, std::min(atoi(e), jb_cap));
     if (!getenv("STC_LDS_BUDGET")) {
         // a thin panel starves the trailing update: take the whole CU instead
         for (const SysDesc& s : sys) {
@@ -172,11 +171,13 @@ int configure(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int refin
     int maxn = 0;
     for (const SysDesc& s : sys) maxn = std::max(maxn, s.n);
     if (maxn < 1 || maxn > 1024) return STC_EINVAL;
-    int rc = plan_systems(sys, ts, adj, &cfg->lds);
-    if (rc) return rc;
     int r = (maxn + 63) / 64, maxr = 1;
     while (maxr < r) maxr <<= 1;
     cfg->maxr = maxr;
+    // the register-resident panel holds 8 / 4 / 2 columns per wavefront
+    const int jb_cap = kWaves * panel_cols_per_wave(refine ? 8 : 16, maxr);
+    int rc = plan_systems(sys, ts, adj, jb_cap, &cfg->lds);
+    if (rc) return rc;
     void* kfun = refine ? pick_kernel<float>(maxr) : pick_kernel<double>(maxr);
     HIP_TRY(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, cfg->lds));
     int per_cu = 0;
@@ -190,7 +191,7 @@ int configure(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int refin
     (void)factor_out;
     size_t slot = 0;
     for (const SysDesc& s : sys)
-        if (!s.lay_small) slot = std::max(slot, (size_t)s.n * s.n);
+        if (!s.lay_small) slot = std::max(slot, slot_elems_for(s.n, s.lay_jb));
     cfg->slot_elems = slot;
     if (slot) {
         rc = h->scratch.ensure(slot * ts * (size_t)cfg->grid);

@@ -92,6 +92,20 @@ constexpr int kChunkCols = 32;    // trailing columns per chunk (kernel: kChunk)
 constexpr int kChunkLdCols = 34;  // padded row length of the U12 chunk in LDS
 constexpr int kSolveBlk = 32;     // diagonal block of the blocked solves
 
+// Panel columns each wavefront keeps in registers (x MAXR rows per lane):
+// about 32 complex values per lane.
+__host__ __device__ constexpr int panel_cols_per_wave(int elem_bytes, int maxr) {
+    return elem_bytes == 8 ? (maxr <= 4 ? 8 : (maxr <= 8 ? 4 : 2))
+                           : (maxr <= 4 ? 4 : (maxr <= 8 ? 2 : 1));
+}
+
+// Elements of a blocked-path slot: the n x n factors plus the inverted
+// diagonal blocks of L and U (ceil(n/jb) blocks of jb x jb each).
+__host__ __device__ inline size_t slot_elems_for(int n, int jb) {
+    const size_t nb = (size_t)((n + jb - 1) / jb);
+    return (size_t)n * n + 2 * nb * jb * jb;
+}
+
 // Fixed region at the start of LDS: ipiv and the row map (n ints each),
 // scalars (16 ints), reduction scratch.
 __host__ __device__ inline int lds_fixed_bytes(int n) {
@@ -120,11 +134,12 @@ inline Layout plan_layout(int n, int nrhs, int ts, int budget, int jb_cap, int a
     }
     // blocked: LDS panel (n x jb) + one U12 chunk (jb x kChunkLdCols); the
     // solve phase reuses the same region for its vectors + a staged diagonal block
-    const int jbs[] = {64, 48, 32, 28, 24, 20, 16, 12, 8, 4};
-    for (int a = 0; a < 10; ++a) {
+    const int jbs[] = {32, 28, 24, 20, 16, 12, 8, 4};
+    for (int a = 0; a < 8; ++a) {
         int jb = jbs[a];
         if (jb > jb_cap) continue;
-        int need = fixed + align16((long long)n * jb * ts) + align16((long long)jb * kChunkLdCols * ts);
+        int need = fixed + align16((long long)n * jb * ts) + align16((long long)jb * kChunkLdCols * ts) +
+                   align16((long long)jb * jb * ts);
         if (need <= budget) {
             L.jb = jb; L.cw = kChunkCols; L.lds = need;
             for (int G = Gmax; G >= 1; --G) {

@@ -225,8 +225,9 @@ __device__ void panel_factor(cx<T>* P, int ldp, int m, int jb, int j0, int* ipiv
 }
 
 // ---- MFMA 16x16x4 (one wavefront): D[i][j] += sum_k A[i][k] B[k][j] ------------------
-// Lane l supplies A[l%16][l/16] and B[l/16][l%16]; D[4*(l/16)+r][l%16] is in
-// accumulator element r.
+// Lane l supplies A[l%16][l/16] and B[l/16][l%16].  Accumulator element r of
+// lane l holds D[i][l%16] with i = 4*(l/16) + r for f32 and i = l/16 + 4*r for
+// f64 (both measured on gfx950); Mfma<T>::row gives i.
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 template <typename T>
@@ -234,6 +235,7 @@ struct Mfma;
 template <>
 struct Mfma<float> {
     using acc = v4f;
+    static __device__ __forceinline__ int row(int lg, int r) { return 4 * lg + r; }
     static __device__ __forceinline__ acc run(float a, float b, acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
@@ -241,6 +243,7 @@ struct Mfma<float> {
 template <>
 struct Mfma<double> {
     using acc = v4d;
+    static __device__ __forceinline__ int row(int lg, int r) { return lg + 4 * r; }
     static __device__ __forceinline__ acc run(double a, double b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
@@ -315,7 +318,7 @@ __device__ void trailing_mfma(cx<T>* __restrict__ S, int ld, const int* rmap, in
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int col = 16 * b + 4 * lg + r;
+                    const int col = 16 * b + M::row(lg, r);
                     cx<T> v = czero<T>();
                     if (rok[a] && col < cw) v = S[prow[a] + (size_t)(cb + col) * ld];
                     cre[a][b][r] = v.re;
@@ -349,7 +352,7 @@ __device__ void trailing_mfma(cx<T>* __restrict__ S, int ld, const int* rmap, in
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int col = 16 * b + 4 * lg + r;
+                    const int col = 16 * b + M::row(lg, r);
                     if (rok[a] && col < cw)
                         S[prow[a] + (size_t)(cb + col) * ld] = cmk<T>(cre[a][b][r], cim[a][b][r]);
                 }
@@ -433,13 +436,246 @@ __device__ __forceinline__ Lds carve(char* smem, int n) {
     return L;
 }
 
+// ---- register-resident panel factorisation (blocked path) --------------------------------
+// Wave w owns panel columns w, w+4, ... (CPW of them) and lane l owns local
+// rows l + 64 t (t < MAXR) of those columns, so the rank-1 updates and the row
+// interchanges never touch memory.  Per column jj: the owning wave searches
+// the pivot (first row of largest |re|+|im|), swaps and scales its column and
+// publishes the multipliers in lcol (LDS); after one barrier every wave swaps
+// rows jj <-> p in its other columns (lane shuffles) and applies the rank-1
+// update.  The factored panel ends in P (LDS, m x jb, ld m) and in S through
+// the final row map.
+template <typename T, int MAXR>
+__device__ __forceinline__ cx<T> reg_get(const cx<T> (&col)[MAXR], int row) {
+    const int slot = row >> 6;
+    cx<T> v = col[0];
+#pragma unroll
+    for (int t = 1; t < MAXR; ++t)
+        if (slot == t) v = col[t];
+    return cshfl(v, row & 63);
+}
+template <typename T, int MAXR>
+__device__ __forceinline__ void reg_set(cx<T> (&col)[MAXR], int row, cx<T> v) {
+    const int slot = row >> 6, dl = row & 63, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int t = 0; t < MAXR; ++t)
+        if (slot == t && lane == dl) col[t] = v;
+}
+
+template <typename T, int MAXR, int CPW>
+__device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, int jb, int* ipiv,
+                                 int* rmap, int* sc, cx<T>* lcol, cx<T>* P) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = n - j0;
+    cx<T> x[CPW][MAXR];
+#pragma unroll
+    for (int q = 0; q < CPW; ++q)
+#pragma unroll
+        for (int t = 0; t < MAXR; ++t) {
+            const int c = wave + kWaves * q, r = lane + 64 * t;
+            x[q][t] = (c < jb && r < m) ? S[(size_t)rmap[j0 + r] + (size_t)(j0 + c) * ld]
+                                        : czero<T>();
+        }
+    for (int jj = 0; jj < jb; ++jj) {
+        const int ow = jj & (kWaves - 1), oq = jj / kWaves;
+        if (wave == ow) {
+            cx<T> col[MAXR];
+#pragma unroll
+            for (int t = 0; t < MAXR; ++t) {
+                col[t] = x[0][t];
+#pragma unroll
+                for (int q = 1; q < CPW; ++q)
+                    if (q == oq) col[t] = x[q][t];
+            }
+            T best = T(-1);
+            int bi = 0x7fffffff;
+#pragma unroll
+            for (int t = 0; t < MAXR; ++t) {
+                const int r = lane + 64 * t;
+                if (r >= jj && r < m) {
+                    const T v = cabs1(col[t]);
+                    if (v > best) { best = v; bi = r; }
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const T ob = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            const int p = bi == 0x7fffffff ? jj : bi;
+            const cx<T> pv = reg_get<T, MAXR>(col, p);
+            const cx<T> vj = reg_get<T, MAXR>(col, jj);
+            const bool zero = pv.re == T(0) && pv.im == T(0);
+            if (!zero) {
+                reg_set<T, MAXR>(col, p, vj);
+                reg_set<T, MAXR>(col, jj, pv);
+                const cx<T> rp = crecip(pv);
+#pragma unroll
+                for (int t = 0; t < MAXR; ++t) {
+                    const int r = lane + 64 * t;
+                    if (r > jj && r < m) col[t] = cmul(col[t], rp);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < MAXR; ++t) {
+                const int r = lane + 64 * t;
+                if (r > jj && r < m) lcol[r] = col[t];
+#pragma unroll
+                for (int q = 0; q < CPW; ++q)
+                    if (q == oq) x[q][t] = col[t];
+            }
+            if (lane == 0) {
+                sc[0] = zero ? jj : p;
+                ipiv[j0 + jj] = j0 + (zero ? jj : p);
+                if (zero && sc[1] == 0) sc[1] = j0 + jj + 1;
+                if (!zero && p != jj) {
+                    const int tmp = rmap[j0 + jj];
+                    rmap[j0 + jj] = rmap[j0 + p];
+                    rmap[j0 + p] = tmp;
+                }
+            }
+        }
+        __syncthreads();
+        const int p = sc[0];
+        if (p != jj) {
+#pragma unroll
+            for (int q = 0; q < CPW; ++q) {
+                const int c = wave + kWaves * q;
+                if (c < jb && c != jj) {
+                    const cx<T> a = reg_get<T, MAXR>(x[q], jj);
+                    const cx<T> b2 = reg_get<T, MAXR>(x[q], p);
+                    reg_set<T, MAXR>(x[q], jj, b2);
+                    reg_set<T, MAXR>(x[q], p, a);
+                }
+            }
+        }
+        cx<T> l[MAXR];
+#pragma unroll
+        for (int t = 0; t < MAXR; ++t) {
+            const int r = lane + 64 * t;
+            l[t] = (r > jj && r < m) ? lcol[r] : czero<T>();
+        }
+#pragma unroll
+        for (int q = 0; q < CPW; ++q) {
+            const int c = wave + kWaves * q;
+            if (c > jj && c < jb) {
+                const cx<T> u = reg_get<T, MAXR>(x[q], jj);
+#pragma unroll
+                for (int t = 0; t < MAXR; ++t) cfms(x[q][t], l[t], u);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < CPW; ++q)
+#pragma unroll
+        for (int t = 0; t < MAXR; ++t) {
+            const int c = wave + kWaves * q, r = lane + 64 * t;
+            if (c < jb && r < m) {
+                P[r + (size_t)c * m] = x[q][t];
+                S[(size_t)rmap[j0 + r] + (size_t)(j0 + c) * ld] = x[q][t];
+            }
+        }
+}
+
+// ---- inverses of the diagonal block L11 (unit lower) and U11 (upper) ----------------------
+// From P[0:jb, 0:jb] (LDS, ld ldp).  Wave w computes columns w, w+4, ... of
+// both, all of its columns interleaved (lanes = rows, jb <= 64).  Linv goes
+// to LDS (ld jb) for the TRSM and, with Uinv, to the slot's block store
+// (block column-major, ld jb) for the triangular solves.
+template <typename T, int CPW>
+__device__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<T>* Linv_lds, cx<T>* __restrict__ Lg,
+                            cx<T>* __restrict__ Ug) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    cx<T> xl[CPW], xu[CPW];
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+        const int j = wave + kWaves * q;
+        xl[q] = lane == j ? cmk<T>(T(1), T(0)) : czero<T>();
+        xu[q] = xl[q];
+    }
+    for (int k = 0; k < jb - 1; ++k) {
+        const cx<T> lk = (lane > k && lane < jb) ? P[lane + (size_t)k * ldp] : czero<T>();
+#pragma unroll
+        for (int q = 0; q < CPW; ++q) cfms(xl[q], lk, cshfl(xl[q], k));
+    }
+    for (int k = jb - 1; k >= 0; --k) {
+        const cx<T> rk = crecip(P[k + (size_t)k * ldp]);
+        const cx<T> uk = (lane < k) ? P[lane + (size_t)k * ldp] : czero<T>();
+#pragma unroll
+        for (int q = 0; q < CPW; ++q) {
+            if (lane == k) xu[q] = cmul(xu[q], rk);
+            cfms(xu[q], uk, cshfl(xu[q], k));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+        const int j = wave + kWaves * q;
+        if (j < jb && lane < jb) {
+            Linv_lds[lane + j * jb] = xl[q];
+            Lg[lane + (size_t)j * jb] = xl[q];
+            Ug[lane + (size_t)j * jb] = xu[q];
+        }
+    }
+}
+
+// ---- U12 chunk = Linv A12 (parallel product) --------------------------------------------------
+// Thread t: column c = t % 32 of the chunk, rows t/32 + 8 s.  A12 comes
+// straight from S (logical rows j0.. via rmap), all jb loads in flight.  The
+// result goes to Uc only: the threads of one column read every row of A12,
+// so S is updated from Uc after the next barrier (uc_to_slot).
+template <typename T, int JB8>
+__device__ void trsm_linv(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb,
+                          const cx<T>* Linv, cx<T>* Uc, int cb, int cw) {
+    const int tid = threadIdx.x, c = tid & 31, i0 = tid >> 5;
+    cx<T> acc[JB8];
+#pragma unroll
+    for (int q = 0; q < JB8; ++q) acc[q] = czero<T>();
+    if (c < cw) {
+        const size_t col = (size_t)(cb + c) * ld;
+        for (int k0 = 0; k0 < jb; k0 += 8) {
+            cx<T> a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                a[u] = (k0 + u < jb) ? S[(size_t)rmap[j0 + k0 + u] + col] : czero<T>();
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + u;
+#pragma unroll
+                for (int q = 0; q < JB8; ++q) {
+                    const int i = i0 + 8 * q;
+                    if (k < jb && i < jb && k <= i) cfma(acc[q], Linv[i + k * jb], a[u]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < JB8; ++q) {
+            const int i = i0 + 8 * q;
+            if (i < jb) Uc[i * kChunkLd + c] = acc[q];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void uc_to_slot(cx<T>* __restrict__ S, int ld, const int* rmap, int j0,
+                                           int jb, const cx<T>* Uc, int cb, int cw) {
+    for (int e = threadIdx.x; e < jb * kChunk; e += kThreads) {
+        const int i = e / kChunk, c = e % kChunk;
+        if (c < cw) S[(size_t)rmap[j0 + i] + (size_t)(cb + c) * ld] = Uc[i * kChunkLd + c];
+    }
+}
+
 // ---- blocked right-looking factorisation in a global slot --------------------------------
 // Rows are never moved in S: a pivot swaps two entries of rmap and the panel
-// rows in LDS, so no row-interchange pass touches global memory.  Row r of
-// the LAPACK-layout factors is physical row rmap[r] of S.
+// rows (registers), so no row-interchange pass touches global memory.  Row r
+// of the LAPACK-layout factors is physical row rmap[r] of S.  The slot also
+// holds the inverted diagonal blocks (Linv, Uinv; jb x jb each per panel),
+// used by the triangular solves.
 template <typename T, int MAXR>
 __device__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* __restrict__ S,
                                const Layout& lay, const Lds& L) {
+    constexpr int CPW = panel_cols_per_wave(sizeof(T), MAXR);
     const int n = s.n, tid = threadIdx.x;
     const size_t nn = (size_t)n * n;
 #pragma unroll 4
@@ -451,30 +687,28 @@ __device__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* __r
     const int jbmax = lay.jb;
     cx<T>* P = reinterpret_cast<cx<T>*>(L.main);
     cx<T>* Uc = reinterpret_cast<cx<T>*>(L.main + align16((long long)n * jbmax * sizeof(cx<T>)));
+    cx<T>* Linv = reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(Uc) +
+                                          align16((long long)jbmax * kChunkLd * sizeof(cx<T>)));
+    cx<T>* lcol = P;  // the multiplier column lives in the panel region while the panel is in registers
+    const size_t nblk = (size_t)((n + jbmax - 1) / jbmax);
+    cx<T>* Lg = S + nn;
+    cx<T>* Ug = Lg + nblk * jbmax * jbmax;
 
-    for (int j0 = 0; j0 < n; j0 += jbmax) {
+    for (int j0 = 0, blk = 0; j0 < n; j0 += jbmax, ++blk) {
         const int jb = min(jbmax, n - j0), m = n - j0;
-        for (int e = tid; e < m * jb; e += kThreads) {
-            const int i = e % m, jj = e / m;
-            P[e] = S[(size_t)L.rmap[j0 + i] + (size_t)(j0 + jj) * n];
-        }
+        panel_factor_reg<T, MAXR, CPW>(S, n, n, j0, jb, L.ipiv, L.rmap, L.sc, lcol, P);
         __syncthreads();
-        panel_factor<T, MAXR>(P, m, m, jb, j0, L.ipiv, L.rmap, L.sc);
-        for (int e = tid; e < m * jb; e += kThreads) {
-            const int i = e % m, jj = e / m;
-            S[(size_t)L.rmap[j0 + i] + (size_t)(j0 + jj) * n] = P[e];
-        }
+        invert_diag<T, CPW>(P, m, jb, Linv, Lg + (size_t)blk * jbmax * jbmax,
+                            Ug + (size_t)blk * jbmax * jbmax);
+        __syncthreads();
         for (int cb = j0 + jb; cb < n; cb += kChunk) {
             const int cw = min(kChunk, n - cb);
-            trsm_chunk(S, n, L.rmap, j0, jb, P, m, Uc, cb, cw);
+            trsm_linv<T, 8>(S, n, L.rmap, j0, jb, Linv, Uc, cb, cw);
             __syncthreads();
-            if constexpr (sizeof(T) == 8)
-                trailing_valu(S, n, L.rmap, j0, jb, m, P, m, Uc, cb, cw);
-            else
-                trailing_mfma(S, n, L.rmap, j0, jb, m, P, m, Uc, cb, cw);
+            uc_to_slot(S, n, L.rmap, j0, jb, Uc, cb, cw);
+            trailing_mfma(S, n, L.rmap, j0, jb, m, P, m, Uc, cb, cw);
             __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -489,6 +723,11 @@ struct TriCtx {
     int n;
     const int* rmap;
     cx<T>* Dg;
+    // blocked path: inverted diagonal blocks (block b at b*bs*bs, ld = its width)
+    const cx<T>* Linv;
+    const cx<T>* Uinv;
+    int bs;
+    int inv_u;  // use Uinv for the U solve (refined mode); else substitution
     __device__ __forceinline__ size_t row(int i) const { return rmap ? (size_t)rmap[i] : (size_t)i; }
     __device__ __forceinline__ cx<T> at(int i, int j) const { return LU[row(i) + (size_t)j * ld]; }
 };
@@ -546,6 +785,57 @@ __device__ __forceinline__ void rect_update(const TriCtx<T>& t, int i, int k0, i
             a.im -= acc[g].im;
             w[i + (size_t)g * t.n] = a;
         }
+}
+
+// y_b = Binv_b w_b for one diagonal block (lanes = rows, one wave per column).
+template <typename T, bool LOWER>
+__device__ __forceinline__ void block_inv_apply(const cx<T>* __restrict__ Bi, int k0, int kb, cx<T>* w,
+                                                int n, int gc) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int g = wave; g < gc; g += kWaves) {
+        cx<T>* y = w + (size_t)g * n + k0;
+        cx<T> acc = czero<T>();
+        for (int k8 = 0; k8 < kb; k8 += 8) {
+            cx<T> bv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = k8 + u;
+                const bool in = lane < kb && k < kb && (LOWER ? k <= lane : k >= lane);
+                bv[u] = in ? Bi[lane + (size_t)k * kb] : czero<T>();
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k8 + u < kb) cfma(acc, bv[u], y[k8 + u]);
+        }
+        if (lane < kb) y[lane] = acc;
+    }
+}
+
+// L unit lower, forward, with the inverted diagonal blocks.
+template <typename T>
+__device__ void lower_unit_solve_inv(const TriCtx<T>& t, cx<T>* w, int gc) {
+    const int tid = threadIdx.x, n = t.n;
+    for (int k0 = 0, b = 0; k0 < n; k0 += t.bs, ++b) {
+        const int kb = min(t.bs, n - k0);
+        block_inv_apply<T, true>(t.Linv + (size_t)b * t.bs * t.bs, k0, kb, w, n, gc);
+        __syncthreads();
+        for (int i = k0 + kb + tid; i < n; i += kThreads) rect_update<T, false, false>(t, i, k0, kb, w, gc);
+        __syncthreads();
+    }
+}
+
+// U upper, backward, with the inverted diagonal blocks.
+template <typename T>
+__device__ void upper_solve_inv(const TriCtx<T>& t, cx<T>* w, int gc) {
+    const int tid = threadIdx.x, n = t.n;
+    const int nb = (n + t.bs - 1) / t.bs;
+    for (int b = nb - 1; b >= 0; --b) {
+        const int k0 = b * t.bs, kb = min(t.bs, n - k0);
+        block_inv_apply<T, false>(t.Uinv + (size_t)b * t.bs * t.bs, k0, kb, w, n, gc);
+        __syncthreads();
+        for (int i = tid; i < k0; i += kThreads) rect_update<T, false, false>(t, i, k0, kb, w, gc);
+        __syncthreads();
+    }
 }
 
 // L unit lower, forward.
@@ -799,8 +1089,10 @@ __device__ int solve_group(const SysDesc& s, double w, double w2, const TriCtx<T
     const int n = s.n, tid = threadIdx.x;
     auto solve_t = [&]() {
         if (!ADJ) {
-            lower_unit_solve(tri, wv, gc);
-            upper_solve(tri, wv, gc);
+            if (tri.Linv) lower_unit_solve_inv(tri, wv, gc);
+            else lower_unit_solve(tri, wv, gc);
+            if (tri.Linv && tri.inv_u) upper_solve_inv(tri, wv, gc);
+            else upper_solve(tri, wv, gc);
         } else {
             upper_h_solve(tri, wv, gc);
             lower_h_unit_solve(tri, wv, gc);
@@ -989,7 +1281,7 @@ __device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, cha
         LU = A;
         solvep = L.main + align16((long long)nn * sizeof(cx<T>));
     } else {
-        if (kp.scratch == nullptr || kp.slot_elems < (size_t)n * n) {
+        if (kp.scratch == nullptr || kp.slot_elems < slot_elems_for(n, lay.jb)) {
             // the launch did not provide a factor slot: report, touch nothing
             if (tid == 0) s.info[fidx] = -2;
             __syncthreads();
@@ -1048,6 +1340,10 @@ __device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, cha
     tri.n = n;
     tri.rmap = lay.small ? nullptr : L.rmap;
     tri.Dg = lay.small ? nullptr : reinterpret_cast<cx<T>*>(after);
+    tri.bs = lay.jb;
+    tri.Linv = lay.small ? nullptr : LU + (size_t)n * n;
+    tri.Uinv = lay.small ? nullptr : LU + (size_t)n * n + (size_t)((n + lay.jb - 1) / lay.jb) * lay.jb * lay.jb;
+    tri.inv_u = refine ? 1 : 0;
     if (kp.flags & kAdjoint) {
         z64* xk = reinterpret_cast<z64*>(reinterpret_cast<char*>(rs) + align16((long long)n * G * 16));
         z64* bv = reinterpret_cast<z64*>(reinterpret_cast<char*>(xk) + align16((long long)n * G * 16));

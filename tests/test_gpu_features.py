@@ -1,0 +1,242 @@
+"""GPU parity of the adjoint, the uneven persistent batch (+ hipGraph plan), the
+full back-transformation (B_R, C_R, transfer functions) and the large-n path.
+
+Oracle: oracle/stc_oracle.c (pinned in tests/test_oracle.py); the closed-form
+modal solution and gradient of the synthetic ROMs (staccato_amd/synth.py).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from staccato_amd import Plan, synth
+from staccato_amd._lib import SysDesc
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+# ---- adjoint (n = 128, eta = 0.02, 2048 frequencies, 1 RHS) -----------------------------
+
+@pytest.mark.parametrize("mode,tol", [("plain", 0.0), ("refined", 1e-10)])
+def test_adjoint_matches_closed_form_and_oracle(handle, mode, tol):
+    c = synth.adjoint_n128()
+    rom, cv = c.rom, c.extra["c"]
+    handle.rom_set(10, rom.K, rom.M, rom.D, rom.eta)
+    J, ds, de, info = handle.adjoint(10, c.freq, c.F, cv, rom.Phi, rom.kdiag, mode=mode, tol=tol)
+    assert (info == 0).all()
+    Jc, dsc, dec = synth.adjoint_closed_form(rom, c.freq, c.F, cv)
+    gate = 1e-8 if mode == "plain" else 1e-6
+    assert abs(J / Jc - 1) < gate
+    assert np.abs(ds - dsc).max() <= gate * np.abs(dsc).max()
+    assert abs(de / dec - 1) < gate
+    sub = slice(0, 2048, 16)
+    Jo, dso, deo, _ = oracle.adjoint(rom.K, rom.M, rom.D, rom.eta, c.freq[sub], c.F, cv, rom.Phi,
+                                     rom.kdiag, nthreads=8)
+    J2, ds2, de2, _ = handle.adjoint(10, c.freq[sub], c.F, cv, rom.Phi, rom.kdiag, mode=mode, tol=tol)
+    assert abs(J2 / Jo - 1) < gate
+    assert np.abs(ds2 - dso).max() <= gate * np.abs(dso).max()
+    assert abs(de2 / deo - 1) < gate
+
+
+def test_adjoint_central_differences(handle):
+    """dJ/ds_k and dJ/deta against central differences (h = 1e-6) of the GPU sweep."""
+    c = synth.adjoint_n128(256)
+    rom, cv = c.rom, c.extra["c"]
+    handle.rom_set(11, rom.K, rom.M, rom.D, rom.eta)
+    J, ds, de, _ = handle.adjoint(11, c.freq, c.F, cv, rom.Phi, rom.kdiag, mode="plain")
+    h = 1e-6
+
+    def Jof(scale, eta):
+        K = (rom.Phi * (scale * rom.kdiag)) @ rom.Phi.T
+        handle.rom_set(12, 0.5 * (K + K.T), rom.M, rom.D, eta)
+        X, info, _, _ = handle.sweep(12, c.freq, c.F, mode="plain", berr=False)
+        return float(np.sum(np.abs(np.einsum("i,fir->fr", cv.conj(), X)) ** 2))
+
+    one = np.ones(rom.n)
+    gmax = np.abs(ds).max()
+    for k in (0, 31, 77, 127):
+        e = np.zeros(rom.n)
+        e[k] = h
+        fd = (Jof(one + e, rom.eta) - Jof(one - e, rom.eta)) / (2 * h)
+        assert abs(fd - ds[k]) <= 1e-5 * gmax, (k, fd, ds[k])
+    fd = (Jof(one, rom.eta + h) - Jof(one, rom.eta - h)) / (2 * h)
+    assert abs(fd - de) <= 1e-6 * abs(de)
+
+
+# ---- uneven batch in one persistent launch ------------------------------------------------
+
+def _batch(scale):
+    cases = synth.uneven_batch(nsys=64, scale=scale)
+    return cases
+
+
+def test_uneven_batch_refined_vs_oracle(handle):
+    cases = _batch(0.02)   # every system, 5..41 frequencies each
+    systems = []
+    for i, c in enumerate(cases):
+        handle.rom_set(100 + i, c.rom.K, c.rom.M, c.rom.D, c.rom.eta)
+        systems.append((100 + i, c.freq, c.F))
+    outs, st = handle.sweep_batch(systems, mode="refined", tol=1e-10)
+    assert st.points == sum(c.freq.size for c in cases)
+    for (X, info, be), c in zip(outs, cases):
+        assert (info == 0).all(), (c.rom.n, info)
+        assert be.max() <= 1e-10, (c.rom.n, be.max())
+        Xc = synth.closed_form(c.rom, c.freq, c.F)
+        kappa = synth.cond2(c.rom, c.freq)
+        fe = np.abs(X - Xc).max(axis=(1, 2)) / np.abs(Xc).max(axis=(1, 2))
+        assert (fe <= 4e-10 * kappa + 1e-13).all(), (c.rom.n, (fe / kappa).max())
+    # the oracle on a few systems of each size class
+    seen = set()
+    for (X, info, be), c in zip(outs, cases):
+        if c.rom.n in seen or c.rom.n > 256:
+            continue
+        seen.add(c.rom.n)
+        Xo, _, _, _ = oracle.sweep(c.rom.K, c.rom.M, c.rom.D, c.rom.eta, c.freq[:4], c.F,
+                                   nthreads=8)
+        assert _rel(X[:4], Xo) < 1e-6
+
+
+def test_plan_graph_replays_match_eager(handle):
+    import torch
+    cases = _batch(0.01)[:12]
+    dev = torch.device("cuda:0")
+    keep, descs = [], (SysDesc * len(cases))()
+    for i, c in enumerate(cases):
+        handle.rom_set(200 + i, c.rom.K, c.rom.M, c.rom.D, c.rom.eta)
+        n, nf, nr = c.rom.n, c.freq.size, c.F.shape[1]
+        fr = torch.tensor(c.freq, dtype=torch.float64, device=dev)
+        F = torch.tensor(np.ascontiguousarray(c.F.T), dtype=torch.complex128, device=dev)
+        X = torch.zeros((nf, nr, n), dtype=torch.complex128, device=dev)
+        info = torch.zeros(nf, dtype=torch.int32, device=dev)
+        be = torch.zeros((nf, nr), dtype=torch.float64, device=dev)
+        keep.append((fr, F, X, info, be))
+        d = descs[i]
+        d.rom_id, d.nfreq, d.freq_hz, d.nrhs = 200 + i, nf, fr.data_ptr(), nr
+        d.F, d.ldf, d.X, d.ldx = F.data_ptr(), n, X.data_ptr(), n
+        d.info, d.berr = info.data_ptr(), be.data_ptr()
+    plan = Plan(handle, descs, mode="refined", tol=1e-10)
+    eager = [handle.sweep(200 + i, c.freq, c.F, mode="refined", tol=1e-10)[0]
+             for i, c in enumerate(cases)]
+    for rep in range(3):
+        for _, _, X, _, _ in keep:
+            X.zero_()
+        torch.cuda.synchronize()
+        plan.execute()
+        torch.cuda.synchronize()
+        for (fr, F, X, info, be), Xe in zip(keep, eager):
+            assert (info.cpu().numpy() == 0).all()
+            assert be.cpu().numpy().max() <= 1e-10
+            Xg = X.cpu().numpy().transpose(0, 2, 1)
+            assert _rel(Xg, Xe) < 1e-9, rep
+        # eager work between replays
+        handle.sweep(200, cases[0].freq, cases[0].F, mode="plain")
+    plan.close()
+
+
+# ---- full back-transformation (KrylovROMSubstructure.cpp:1393-1460) ------------------------
+
+@pytest.mark.parametrize("mode", ["plain", "refined"])
+def test_back_transform_outputs_and_transfer_functions(handle, mode):
+    rom, rng = synth.modal_rom(96, 21, 0.02, alpha=1e-4, beta=0.2)
+    nin, nout, nrhs = 6, 5, 3
+    B = synth.complex_normal(rng, (96, nin))
+    C = synth.complex_normal(rng, (nout, 96))
+    load = synth.complex_normal(rng, (nin, nrhs))
+    freq = np.linspace(3.0, 1500.0, 40)
+    handle.rom_set(20, rom.K, rom.M, rom.D, rom.eta)
+    handle.rom_set_io(20, B, C)
+    Y, H, info, st = handle.back_transform(20, freq, load, nin, nout, transfer=True, mode=mode,
+                                           tol=1e-10)
+    assert (info == 0).all()
+    Xc = synth.closed_form(rom, freq, B @ load)
+    Yc = np.einsum("oi,fir->for", C, Xc)
+    Hc = np.einsum("oi,fir->for", C, synth.closed_form(rom, freq, B))
+    kappa = synth.cond2(rom, freq).max()
+    gate = (1e-13 if mode == "plain" else 4e-10) * kappa
+    assert _rel(Y, Yc) < gate
+    assert _rel(H, Hc) < gate
+    # Y = H inputLoad (the transfer function reproduces the response)
+    assert _rel(np.einsum("foi,ir->for", H, load), Y) < gate
+
+
+# ---- large dense ROM (n = 1024, 16 RHS) ------------------------------------------------------
+
+def test_n1024_subset_refined_and_plain(handle):
+    c = synth.dense_n1024()
+    freq = c.freq[::64][:8]
+    handle.rom_set(30, c.rom.K, c.rom.M, c.rom.D, c.rom.eta)
+    Xc = synth.closed_form(c.rom, freq, c.F)
+    kappa = synth.cond2(c.rom, freq)
+    for mode, tol, fgate in (("refined", 1e-10, 4e-10), ("plain", 1e-12, 1e-12)):
+        X, info, be, st = handle.sweep(30, freq, c.F, mode=mode, tol=tol)
+        assert (info == 0).all(), (mode, info)
+        assert be.max() <= tol, (mode, be.max())
+        fe = np.abs(X - Xc).max(axis=(1, 2)) / np.abs(Xc).max(axis=(1, 2))
+        assert (fe <= fgate * kappa + 1e-13).all(), (mode, (fe / kappa).max())
+    Xo, _, _, _ = oracle.sweep(c.rom.K, c.rom.M, c.rom.D, c.rom.eta, freq[:1], c.F, nthreads=8)
+    assert _rel(X[:1], Xo) < 1e-9
+
+
+def test_n256_full_sweep_properties(handle):
+    """All 4096 frequencies: every point meets the backward-error gate and the
+    forward error obeys kappa * tol (closed form), both modes."""
+    c = synth.uniform_n256()
+    handle.rom_set(31, c.rom.K, c.rom.M, c.rom.D, c.rom.eta)
+    Xc = synth.closed_form(c.rom, c.freq, c.F)
+    kappa = synth.cond2(c.rom, c.freq)
+    for mode, tol, fgate in (("refined", 1e-10, 4e-10), ("plain", 1e-12, 1e-13)):
+        X, info, be, st = handle.sweep(31, c.freq, c.F, mode=mode, tol=tol)
+        assert (info == 0).all()
+        assert be.max() <= tol
+        fe = np.abs(X - Xc).max(axis=(1, 2)) / np.abs(Xc).max(axis=(1, 2))
+        assert (fe <= fgate * kappa + 1e-13).all(), (mode, (fe / kappa).max())
+
+
+def test_ld_and_empty_inputs(handle):
+    rom, rng = synth.modal_rom(40, 9, 0.02)
+    handle.rom_set(32, rom.K, rom.M, rom.D, rom.eta)
+    F = synth.complex_normal(rng, (40, 2))
+    X, info, be, st = handle.sweep(32, np.array([]), F, mode="plain")
+    assert X.shape == (0, 40, 2)
+    X1, info1, _, _ = handle.sweep(32, np.array([10.0, 20.0]), F, mode="plain")
+    X2, info2, _, _ = handle.sweep(32, np.array([10.0, 20.0]), F[:, :1], mode="plain")
+    np.testing.assert_allclose(X2[:, :, 0], X1[:, :, 0], rtol=1e-13)
+
+
+# ---- C++ host mirror: KrylovROMSweep::backTransformKMOR -----------------------------------
+
+def test_cpp_back_transform_kmor_writes_reference_format(handle, tmp_path):
+    import ctypes
+    from staccato_amd import _lib
+    host = ctypes.CDLL(_lib.LIB_PATH.replace("libstaccato_sweep.so", "libstaccato_host.so"))
+    f = host.stc_host_back_transform_kmor
+    f.restype = ctypes.c_int
+    rom, rng = synth.modal_rom(48, 13, 0.0, alpha=1e-4, beta=0.1)
+    nin, nout, nload = 3, 4, 2
+    eta = 0.02
+    Keta = (1 + 1j * eta) * rom.K      # the reference's K_R carries (1 + i eta)
+    B = synth.complex_normal(rng, (48, nin))
+    C = synth.complex_normal(rng, (nout, 48))
+    load = synth.complex_normal(rng, (nin, nload))
+    freq = np.linspace(5.0, 800.0, 12)
+    arrs = [np.asfortranarray(a, dtype=np.complex128) for a in (Keta, rom.M, rom.D, B, C, load)]
+    res = np.zeros(12 * nout * nload, dtype=np.complex128)
+    H = np.zeros(12 * nout * nin, dtype=np.complex128)
+    info = np.zeros(12, dtype=np.int32)
+    name = str(tmp_path / "beam")
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)
+    rc = f(0, 48, p(arrs[0]), p(arrs[1]), p(arrs[2]), nin, p(arrs[3]), nout, p(arrs[4]), 12,
+           p(freq), p(arrs[5]), nload, name.encode(), 0, p(res), p(H), p(info))
+    assert rc == 0 and (info == 0).all()
+    rom.eta = eta
+    Yc = np.einsum("oi,fir->fro", C, synth.closed_form(rom, freq, B @ load))   # [f][rhs][out]
+    kappa = synth.cond2(rom, freq).max()
+    assert _rel(res.reshape(12, nload, nout), Yc) < 1e-13 * kappa
+    lines = open(name + "_KMOR_Results.dat").read().split("\n")
+    assert len([l for l in lines if l]) == res.size
+    re, im = lines[0].split("\t")
+    assert float(re) == res[0].real and float(im) == res[0].imag
+    assert "e" in re and len(re.split("e")[0].split(".")[1]) == 16
