@@ -69,6 +69,80 @@ void KrylovROMSweep::writeMKLComplexVectorDatFormat(const std::string& fileName,
     f << std::endl;
 }
 
+int KrylovROMSweep::readMtxDense(const std::string& fileName, std::vector<STACCATOComplexDouble>& A,
+                                 int& rows, int& cols, bool isComplex) {
+    std::ifstream in(fileName);
+    if (!in) return STC_EINVAL;
+    while (in.peek() == '%') in.ignore(1 << 20, '\n');
+    long long r = 0, c = 0;
+    if (!(in >> r >> c) || r < 0 || c < 0 || r * c > (1LL << 34)) return STC_EINVAL;
+    A.assign((size_t)(r * c), {0.0, 0.0});
+    for (size_t e = 0; e < A.size(); ++e) {
+        double re, im;
+        if (!(in >> re >> im)) return STC_EINVAL;
+        A[e] = {re, isComplex ? im : 0.0};
+    }
+    rows = (int)r;
+    cols = (int)c;
+    return STC_OK;
+}
+
+void KrylovROMSweep::writeMKLComplexDenseMatrixMtxFormat(const std::string& fileName,
+                                                         const std::vector<STACCATOComplexDouble>& A,
+                                                         int rows, int cols) {
+    std::ofstream f(fileName);
+    f.precision(std::numeric_limits<double>::digits10 + 10);
+    f << std::scientific;
+    f << "%%MatrixMarket matrix coordinate complex dense column major" << std::endl;
+    f << rows << "\t" << cols << std::endl;
+    for (const auto& z : A) f << z.real << "\t" << z.imag << std::endl;
+    f << std::endl;
+}
+
+int KrylovROMSweep::importROMFromFiles(const std::string& prefix) {
+    int r = 0, c = 0;
+    if (readMtxDense(prefix + "_myKR.dat", myKComplexReduced, r, c) || r != c) return STC_EINVAL;
+    const int n = r;
+    if (readMtxDense(prefix + "_myMR.dat", myMComplexReduced, r, c) || r != n || c != n)
+        return STC_EINVAL;
+    std::ifstream probe(prefix + "_myDR.dat");
+    if (probe) {
+        if (readMtxDense(prefix + "_myDR.dat", myDComplexReduced, r, c) || r != n || c != n)
+            return STC_EINVAL;
+    } else {
+        myDComplexReduced.clear();
+    }
+    if (readMtxDense(prefix + "_myBR.dat", myBReduced, r, c) || r != n) return STC_EINVAL;
+    numInputs = c;
+    if (readMtxDense(prefix + "_myCR.dat", myCReduced, r, c) || c != n) return STC_EINVAL;
+    numOutputs = r;
+    ROM_DOF = n;
+    myUploaded = false;
+    return STC_OK;
+}
+
+// File-format entry for tests and non-C++ callers: read one dense MTX file
+// into out (capacity cap complex values); rows/cols receive the shape.
+extern "C" int stc_host_read_mtx_dense(const char* path, int is_complex, int* rows, int* cols,
+                                       stc_z64* out, long long cap) {
+    std::vector<STACCATOComplexDouble> A;
+    int r = 0, c = 0;
+    const int rc = KrylovROMSweep::readMtxDense(path, A, r, c, is_complex != 0);
+    if (rc) return rc;
+    *rows = r;
+    *cols = c;
+    if ((long long)A.size() > cap) return STC_EINVAL;
+    std::copy(A.begin(), A.end(), reinterpret_cast<STACCATOComplexDouble*>(out));
+    return STC_OK;
+}
+
+extern "C" int stc_host_write_mtx_dense(const char* path, int rows, int cols, const stc_z64* A) {
+    const auto* p = reinterpret_cast<const STACCATOComplexDouble*>(A);
+    KrylovROMSweep::writeMKLComplexDenseMatrixMtxFormat(
+        path, std::vector<STACCATOComplexDouble>(p, p + (size_t)rows * cols), rows, cols);
+    return STC_OK;
+}
+
 // C entry for tests and non-C++ callers: one back-transformation through the
 // class above (operators column-major; results, H as in stc_back_transform).
 extern "C" int stc_host_back_transform_kmor(int device, int n, const stc_z64* K, const stc_z64* M,

@@ -60,6 +60,22 @@ public:
     static void writeMKLComplexVectorDatFormat(const std::string& fileName,
                                                const std::vector<STACCATOComplexDouble>& v);
 
+    // Dense "MatrixMarket" text of the ROM export (AuxiliaryFunctions.cpp:80-110,
+    // column-major branch): '%' header lines, "rows<TAB>cols", then one
+    // "re<TAB>im" line per entry in column-major order.  readMtxDense reads
+    // exactly rows*cols entries (isComplex = false drops the imaginary parts,
+    // as srcPerformanceTest/io/io.cpp:17-73 does for real operators) and
+    // returns STC_OK, or STC_EINVAL for a missing file or a short/garbled body.
+    static int readMtxDense(const std::string& fileName, std::vector<STACCATOComplexDouble>& A,
+                            int& rows, int& cols, bool isComplex = true);
+    static void writeMKLComplexDenseMatrixMtxFormat(const std::string& fileName,
+                                                    const std::vector<STACCATOComplexDouble>& A,
+                                                    int rows, int cols);
+    // Reads <prefix>_myKR.dat, _myMR.dat, _myDR.dat (optional), _myBR.dat and
+    // _myCR.dat, the files KrylovROMSubstructure::exportROMToFiles writes
+    // (KrylovROMSubstructure.cpp:1055-1063), and sets the ROM members.
+    int importROMFromFiles(const std::string& prefix);
+
 private:
     int upload();
     stc_handle myHandle = nullptr;
