@@ -120,6 +120,80 @@ def cpu_baseline(case, budget_s=20.0):
                       f"OpenMP over frequencies, 1 LAPACK thread each, {t:.1f} s"}
 
 
+def other_workload(name, steps, warmup):
+    """Throughput of the other configurations (reported beside the headline,
+    never as its value): the uneven 64-ROM batch (hipGraph plan), the n=128
+    adjoint and the n=1024 dense sweep, all refined FP32 unless stated."""
+    import torch
+    from staccato_amd import Handle, Plan, synth
+    from staccato_amd._lib import SysDesc
+    dev = torch.device("cuda:0")
+    h = Handle(0)
+    out = {"workload": name}
+    if name == "uneven":
+        cases = synth.uneven_batch(nsys=64)
+        keep, descs = [], (SysDesc * len(cases))()
+        W = 0.0
+        for i, c in enumerate(cases):
+            h.rom_set(i, c.rom.K, c.rom.M, c.rom.D, c.rom.eta)
+            n, nf, nr = c.rom.n, c.freq.size, c.F.shape[1]
+            t = (torch.tensor(c.freq, dtype=torch.float64, device=dev),
+                 torch.tensor(np.ascontiguousarray(c.F.T), dtype=torch.complex128, device=dev),
+                 torch.empty((nf, nr, n), dtype=torch.complex128, device=dev),
+                 torch.empty(nf, dtype=torch.int32, device=dev),
+                 torch.empty((nf, nr), dtype=torch.float64, device=dev))
+            keep.append(t)
+            d = descs[i]
+            d.rom_id, d.nfreq, d.freq_hz, d.nrhs = i, nf, t[0].data_ptr(), nr
+            d.F, d.ldf, d.X, d.ldx = t[1].data_ptr(), n, t[2].data_ptr(), n
+            d.info, d.berr = t[3].data_ptr(), t[4].data_ptr()
+            W += nf * work_per_point(n, nr)
+        plan = Plan(h, descs, mode="refined", tol=TOL_REFINED)
+        stream = torch.cuda.current_stream()
+        wall, _ = time_gpu(lambda: plan.execute(), steps, warmup, stream)
+        pts = sum(c.freq.size for c in cases)
+        bmax = max(float(t[4].max()) for t in keep)
+        bad = sum(int((t[3] != 0).sum()) for t in keep)
+        out.update(points=pts, pts_per_s=pts * steps / wall, tflops=W * steps / wall / 1e12,
+                   max_backward_error=bmax, points_failed=bad)
+        plan.close()
+    elif name == "adjoint":
+        c = synth.adjoint_n128()
+        rom = c.rom
+        h.rom_set(0, rom.K, rom.M, rom.D, rom.eta)
+        for mode in ("plain", "refined"):
+            for _ in range(warmup):
+                h.adjoint(0, c.freq, c.F, c.extra["c"], rom.Phi, rom.kdiag, mode=mode, tol=TOL_REFINED)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                J, ds, de, info = h.adjoint(0, c.freq, c.F, c.extra["c"], rom.Phi, rom.kdiag,
+                                            mode=mode, tol=TOL_REFINED)
+            dt = (time.perf_counter() - t0) / steps
+            Jc, dsc, dec = synth.adjoint_closed_form(rom, c.freq, c.F, c.extra["c"])
+            out[mode] = {"pts_per_s_incl_host_copies": c.freq.size / dt,
+                         "grad_rel_err_vs_closed_form": float(np.abs(ds - dsc).max() / np.abs(dsc).max()),
+                         "deta_rel_err": float(abs(de / dec - 1))}
+    elif name == "dense1024":
+        c = synth.dense_n1024()
+        rom = c.rom
+        h.rom_set(0, rom.K, rom.M, rom.D, rom.eta)
+        fr = torch.tensor(c.freq, dtype=torch.float64, device=dev)
+        F = torch.tensor(np.ascontiguousarray(c.F.T), dtype=torch.complex128, device=dev)
+        X = torch.empty((c.freq.size, 16, 1024), dtype=torch.complex128, device=dev)
+        info = torch.empty(c.freq.size, dtype=torch.int32, device=dev)
+        be = torch.empty((c.freq.size, 16), dtype=torch.float64, device=dev)
+        stream = torch.cuda.current_stream()
+        for mode in ("refined", "plain"):
+            run = lambda: h.sweep_dev(0, fr, F, X, info, be if mode == "refined" else None,
+                                      mode=mode, tol=TOL_REFINED, max_iter=10,
+                                      stream=stream.cuda_stream)
+            wall, _ = time_gpu(run, steps, warmup, stream)
+            out[mode] = {"pts_per_s": c.freq.size * steps / wall,
+                         "tflops": c.freq.size * work_per_point(1024, 16) * steps / wall / 1e12}
+    h.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +202,9 @@ def main():
     ap.add_argument("--no-vendor", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="both", choices=["both", "refined", "plain"])
+    ap.add_argument("--workload", default="uniform",
+                    choices=["uniform", "uneven", "adjoint", "dense1024"],
+                    help="uniform = the headline n=256 sweep; the others print their own line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (a kernel of known "
                          "traffic, to scale rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -137,6 +214,10 @@ def main():
               "(the sweep shards by frequency with no exchange; see the design notes)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "uniform":
+        other_workload(args.workload, args.steps, args.warmup)
+        return
 
     import torch
     from staccato_amd import Handle, synth

@@ -476,6 +476,8 @@ __device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, i
             x[q][t] = (c < jb && r < m) ? S[(size_t)rmap[j0 + r] + (size_t)(j0 + c) * ld]
                                         : czero<T>();
         }
+    // every wave has gathered through rmap before the first pivot swaps it
+    __syncthreads();
     for (int jj = 0; jj < jb; ++jj) {
         const int ow = jj & (kWaves - 1), oq = jj / kWaves;
         if (wave == ow) {
@@ -675,7 +677,7 @@ __device__ __forceinline__ void uc_to_slot(cx<T>* __restrict__ S, int ld, const 
 template <typename T, int MAXR>
 __device__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* __restrict__ S,
                                const Layout& lay, const Lds& L) {
-    constexpr int CPW = panel_cols_per_wave(sizeof(T), MAXR);
+    constexpr int CPW = panel_cols_per_wave((int)sizeof(cx<T>), MAXR);
     const int n = s.n, tid = threadIdx.x;
     const size_t nn = (size_t)n * n;
 #pragma unroll 4
@@ -1281,8 +1283,9 @@ __device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, cha
         LU = A;
         solvep = L.main + align16((long long)nn * sizeof(cx<T>));
     } else {
-        if (kp.scratch == nullptr || kp.slot_elems < slot_elems_for(n, lay.jb)) {
-            // the launch did not provide a factor slot: report, touch nothing
+        if (kp.scratch == nullptr || kp.slot_elems < slot_elems_for(n, lay.jb) ||
+            lay.jb > kWaves * panel_cols_per_wave((int)sizeof(cx<T>), MAXR) || lay.jb > 64) {
+            // no factor slot, or a panel wider than the register panel: report, touch nothing
             if (tid == 0) s.info[fidx] = -2;
             __syncthreads();
             return;
