@@ -86,6 +86,26 @@ template <typename T>
 __device__ __forceinline__ cx<T> cshfl(cx<T> z, int src) {
     return cmk<T>(__shfl(z.re, src, 64), __shfl(z.im, src, 64));
 }
+// Broadcast from a wavefront-uniform lane: v_readlane into a scalar register
+// (no LDS crossbar traffic, a few cycles of latency).
+__device__ __forceinline__ float lane_read(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_read(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cbcast(cx<T> z, int l) {
+    return cmk<T>(lane_read(z.re, l), lane_read(z.im, l));
+}
+// Replace lane l of a register with the (uniform) value v.
+template <typename T>
+__device__ __forceinline__ cx<T> cput(cx<T> v, int l, cx<T> old) {
+    return ((int)(threadIdx.x & 63) == l) ? v : old;
+}
 template <typename T>
 __device__ __forceinline__ cx<T> to_t(z64 a) { return cmk<T>((T)a.re, (T)a.im); }
 template <typename T>
@@ -274,7 +294,7 @@ __device__ void trsm_chunk(cx<T>* __restrict__ S, int ld, const int* rmap, int j
         const cx<T> lk = (lane > k && lane < jb) ? P[lane + (size_t)k * ldp] : czero<T>();
 #pragma unroll
         for (int q = 0; q < CPW; ++q) {
-            const cx<T> vk = cshfl(y[q], k);
+            const cx<T> vk = cbcast(y[q], k);
             cfms(y[q], lk, vk);
         }
     }
@@ -454,30 +474,35 @@ __device__ __forceinline__ cx<T> reg_get(const cx<T> (&col)[MAXR], int row) {
         if (slot == t) v = col[t];
     return cshfl(v, row & 63);
 }
-template <typename T, int MAXR>
-__device__ __forceinline__ void reg_set(cx<T> (&col)[MAXR], int row, cx<T> v) {
-    const int slot = row >> 6, dl = row & 63, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int t = 0; t < MAXR; ++t)
-        if (slot == t && lane == dl) col[t] = v;
-}
 
+// Register panel without row interchanges.  Rows stay where they were
+// gathered (gather row g = lane + 64 t); every lane tracks lpos[t], the LAPACK
+// logical position of its rows (a pivot swaps the positions jj <-> p only).
+// Pivot candidates at step jj are the rows with lpos >= jj; ties go to the
+// smallest lpos (first index, as izamax).  At the end the panel is scattered
+// once into logical order (P, LDS) and into S at the rows' unchanged physical
+// addresses, and rmap is rewritten: rmap[j0 + lpos] = physical row.
 template <typename T, int MAXR, int CPW>
 __device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, int jb, int* ipiv,
                                  int* rmap, int* sc, cx<T>* lcol, cx<T>* P) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = n - j0;
     cx<T> x[CPW][MAXR];
+    int lpos[MAXR];
+    int prow[MAXR];
+#pragma unroll
+    for (int t = 0; t < MAXR; ++t) {
+        const int r = lane + 64 * t;
+        lpos[t] = r < m ? r : 0x7fffffff;
+        prow[t] = r < m ? rmap[j0 + r] : 0;
+    }
 #pragma unroll
     for (int q = 0; q < CPW; ++q)
 #pragma unroll
         for (int t = 0; t < MAXR; ++t) {
             const int c = wave + kWaves * q, r = lane + 64 * t;
-            x[q][t] = (c < jb && r < m) ? S[(size_t)rmap[j0 + r] + (size_t)(j0 + c) * ld]
-                                        : czero<T>();
+            x[q][t] = (c < jb && r < m) ? S[(size_t)prow[t] + (size_t)(j0 + c) * ld] : czero<T>();
         }
-    // every wave has gathered through rmap before the first pivot swaps it
-    __syncthreads();
     for (int jj = 0; jj < jb; ++jj) {
         const int ow = jj & (kWaves - 1), oq = jj / kWaves;
         if (wave == ow) {
@@ -489,80 +514,80 @@ __device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, i
                 for (int q = 1; q < CPW; ++q)
                     if (q == oq) col[t] = x[q][t];
             }
+            // argmax of |re|+|im| over rows with lpos >= jj; ties: smallest lpos
             T best = T(-1);
-            int bi = 0x7fffffff;
+            int bpos = 0x7fffffff, bg = 0;
 #pragma unroll
             for (int t = 0; t < MAXR; ++t) {
-                const int r = lane + 64 * t;
-                if (r >= jj && r < m) {
+                if (lpos[t] >= jj && lpos[t] != 0x7fffffff) {
                     const T v = cabs1(col[t]);
-                    if (v > best) { best = v; bi = r; }
+                    if (v > best || (v == best && lpos[t] < bpos)) {
+                        best = v;
+                        bpos = lpos[t];
+                        bg = lane + 64 * t;
+                    }
                 }
             }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
                 const T ob = __shfl_xor(best, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+                const int op = __shfl_xor(bpos, off, 64);
+                const int og = __shfl_xor(bg, off, 64);
+                if (ob > best || (ob == best && op < bpos)) { best = ob; bpos = op; bg = og; }
             }
-            const int p = bi == 0x7fffffff ? jj : bi;
-            const cx<T> pv = reg_get<T, MAXR>(col, p);
-            const cx<T> vj = reg_get<T, MAXR>(col, jj);
+            const cx<T> pv = reg_get<T, MAXR>(col, bg);
             const bool zero = pv.re == T(0) && pv.im == T(0);
-            if (!zero) {
-                reg_set<T, MAXR>(col, p, vj);
-                reg_set<T, MAXR>(col, jj, pv);
+            int pl = bpos, pg = bg;
+            if (zero) {
+                // no pivot: the row at logical position jj stays (LAPACK: ipiv = jj)
+                pl = jj;
+                int g = 0x7fffffff;
+#pragma unroll
+                for (int t = 0; t < MAXR; ++t)
+                    if (lpos[t] == jj) g = lane + 64 * t;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) g = min(g, __shfl_xor(g, off, 64));
+                pg = g;
+            } else {
                 const cx<T> rp = crecip(pv);
 #pragma unroll
                 for (int t = 0; t < MAXR; ++t) {
-                    const int r = lane + 64 * t;
-                    if (r > jj && r < m) col[t] = cmul(col[t], rp);
+                    // rows below the pivot after this step: lpos > jj, except the pivot row
+                    const bool below = lpos[t] >= jj && lpos[t] != 0x7fffffff && (lane + 64 * t) != pg;
+                    if (below) col[t] = cmul(col[t], rp);
                 }
             }
 #pragma unroll
             for (int t = 0; t < MAXR; ++t) {
-                const int r = lane + 64 * t;
-                if (r > jj && r < m) lcol[r] = col[t];
+                const bool below = lpos[t] >= jj && lpos[t] != 0x7fffffff && (lane + 64 * t) != pg;
+                lcol[lane + 64 * t] = below ? col[t] : czero<T>();
 #pragma unroll
                 for (int q = 0; q < CPW; ++q)
                     if (q == oq) x[q][t] = col[t];
             }
             if (lane == 0) {
-                sc[0] = zero ? jj : p;
-                ipiv[j0 + jj] = j0 + (zero ? jj : p);
+                sc[0] = pg;
+                sc[2] = pl;
+                ipiv[j0 + jj] = j0 + pl;
                 if (zero && sc[1] == 0) sc[1] = j0 + jj + 1;
-                if (!zero && p != jj) {
-                    const int tmp = rmap[j0 + jj];
-                    rmap[j0 + jj] = rmap[j0 + p];
-                    rmap[j0 + p] = tmp;
-                }
             }
         }
         __syncthreads();
-        const int p = sc[0];
-        if (p != jj) {
+        const int pg = sc[0], pl = sc[2];
+        // logical positions: jj <-> pl
 #pragma unroll
-            for (int q = 0; q < CPW; ++q) {
-                const int c = wave + kWaves * q;
-                if (c < jb && c != jj) {
-                    const cx<T> a = reg_get<T, MAXR>(x[q], jj);
-                    const cx<T> b2 = reg_get<T, MAXR>(x[q], p);
-                    reg_set<T, MAXR>(x[q], jj, b2);
-                    reg_set<T, MAXR>(x[q], p, a);
-                }
-            }
+        for (int t = 0; t < MAXR; ++t) {
+            if (lpos[t] == jj) lpos[t] = pl;
+            else if (lpos[t] == pl) lpos[t] = jj;
         }
         cx<T> l[MAXR];
 #pragma unroll
-        for (int t = 0; t < MAXR; ++t) {
-            const int r = lane + 64 * t;
-            l[t] = (r > jj && r < m) ? lcol[r] : czero<T>();
-        }
+        for (int t = 0; t < MAXR; ++t) l[t] = lcol[lane + 64 * t];
 #pragma unroll
         for (int q = 0; q < CPW; ++q) {
             const int c = wave + kWaves * q;
             if (c > jj && c < jb) {
-                const cx<T> u = reg_get<T, MAXR>(x[q], jj);
+                const cx<T> u = reg_get<T, MAXR>(x[q], pg);
 #pragma unroll
                 for (int t = 0; t < MAXR; ++t) cfms(x[q][t], l[t], u);
             }
@@ -570,13 +595,18 @@ __device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, i
         __syncthreads();
     }
 #pragma unroll
+    for (int t = 0; t < MAXR; ++t) {
+        const int r = lane + 64 * t;
+        if (r < m) rmap[j0 + lpos[t]] = prow[t];
+    }
+#pragma unroll
     for (int q = 0; q < CPW; ++q)
 #pragma unroll
         for (int t = 0; t < MAXR; ++t) {
             const int c = wave + kWaves * q, r = lane + 64 * t;
             if (c < jb && r < m) {
-                P[r + (size_t)c * m] = x[q][t];
-                S[(size_t)rmap[j0 + r] + (size_t)(j0 + c) * ld] = x[q][t];
+                P[lpos[t] + (size_t)c * m] = x[q][t];
+                S[(size_t)prow[t] + (size_t)(j0 + c) * ld] = x[q][t];
             }
         }
 }
@@ -600,7 +630,7 @@ __device__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<T>* Linv_lds, cx
     for (int k = 0; k < jb - 1; ++k) {
         const cx<T> lk = (lane > k && lane < jb) ? P[lane + (size_t)k * ldp] : czero<T>();
 #pragma unroll
-        for (int q = 0; q < CPW; ++q) cfms(xl[q], lk, cshfl(xl[q], k));
+        for (int q = 0; q < CPW; ++q) cfms(xl[q], lk, cbcast(xl[q], k));
     }
     for (int k = jb - 1; k >= 0; --k) {
         const cx<T> rk = crecip(P[k + (size_t)k * ldp]);
@@ -608,7 +638,7 @@ __device__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<T>* Linv_lds, cx
 #pragma unroll
         for (int q = 0; q < CPW; ++q) {
             if (lane == k) xu[q] = cmul(xu[q], rk);
-            cfms(xu[q], uk, cshfl(xu[q], k));
+            cfms(xu[q], uk, cbcast(xu[q], k));
         }
     }
 #pragma unroll
@@ -853,7 +883,7 @@ __device__ void lower_unit_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
             cx<T>* y = w + (size_t)g * n + k0;
             cx<T> v = lane < kb ? y[lane] : czero<T>();
             for (int k = 0; k < kb - 1; ++k) {
-                const cx<T> vk = cshfl(v, k);
+                const cx<T> vk = cbcast(v, k);
                 if (lane > k && lane < kb) cfms(v, Db[lane + (size_t)k * ldd], vk);
             }
             if (lane < kb) y[lane] = v;
@@ -879,7 +909,7 @@ __device__ void upper_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
             cx<T> v = lane < kb ? y[lane] : czero<T>();
             for (int k = kb - 1; k >= 0; --k) {
                 if (lane == k) v = cdiv(v, Db[k + (size_t)k * ldd]);
-                const cx<T> vk = cshfl(v, k);
+                const cx<T> vk = cbcast(v, k);
                 if (lane < k) cfms(v, Db[lane + (size_t)k * ldd], vk);
             }
             if (lane < kb) y[lane] = v;
@@ -904,7 +934,7 @@ __device__ void upper_h_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
             cx<T> v = lane < kb ? y[lane] : czero<T>();
             for (int k = 0; k < kb; ++k) {
                 if (lane == k) v = cdiv(v, cconj(Db[k + (size_t)k * ldd]));
-                const cx<T> vk = cshfl(v, k);
+                const cx<T> vk = cbcast(v, k);
                 if (lane > k && lane < kb) cfms(v, cconj(Db[k + (size_t)lane * ldd]), vk);
             }
             if (lane < kb) y[lane] = v;
@@ -929,7 +959,7 @@ __device__ void lower_h_unit_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
             cx<T>* y = w + (size_t)g * n + k0;
             cx<T> v = lane < kb ? y[lane] : czero<T>();
             for (int k = kb - 1; k > 0; --k) {
-                const cx<T> vk = cshfl(v, k);
+                const cx<T> vk = cbcast(v, k);
                 if (lane < k) cfms(v, cconj(Db[k + (size_t)lane * ldd]), vk);
             }
             if (lane < kb) y[lane] = v;
