@@ -66,6 +66,14 @@ def load(path=None):
     if _lib is not None:
         return _lib
     p = path or LIB_PATH
+    # torch ships its own libamdhip64.so.7 (same SONAME as /opt/rocm's).  The
+    # first copy loaded serves the whole process, and torch does not start on
+    # a newer runtime than the one it was built with, while this library runs
+    # on either: let torch's copy load first when torch is installed.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(p):
         raise OSError(
             f"{p} is missing: build it with staccato_amd.build.build_library() "
