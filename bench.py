@@ -11,7 +11,8 @@ resident in HBM before the timed region; outputs stay on the device.
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 Prints one JSON line.  The plain FP64 sweep (the denominator of the metric),
-a vendor batched LU (torch.linalg on the materialised A(w), complex128) and the
+a vendor batched LU (rocSOLVER zgetrf/zgetrs strided-batched on the
+materialised A(w), complex128) and the
 CPU reference loop (zgetrf + zgetrs per frequency over OpenBLAS, OpenMP over
 frequencies) are measured beside it.
 """
@@ -31,6 +32,13 @@ TOL_REFINED, TOL_PLAIN = 1e-10, 1e-12
 PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the sweep kernel on this workload: rocprofv3 --pmc passes
+# (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean of 4 launches) scaled by what
+# the same counters read for a 256 MiB device-to-device copy (FETCH_SIZE reads half
+# the bytes copied: x2.0; WRITE_SIZE reads them exactly: x1.0).  The summary and the
+# commands are in profiles/sweep_kernel_n256_pmc.txt; re-measure when the kernel changes.
+TRAFFIC_KIB = {"refined": (7.507e6, 11.024e6), "plain": (16.53e6, 30.77e6)}
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
 def work_per_point(n, nrhs):
@@ -57,28 +65,70 @@ def time_gpu(run, steps, warmup, stream):
 
 
 def vendor_fp64(case, steps, warmup):
-    """Plain FP64 with the vendor stack: A(w) materialised for every frequency,
-    torch.linalg.lu_factor + lu_solve (complex128, batched)."""
+    """Plain FP64 with the vendor stack: A(w) materialised per frequency (torch
+    elementwise ops), rocsolver_zgetrf_strided_batched + zgetrs_strided_batched
+    (librocsolver, called through ctypes), in chunks of 512 frequencies."""
+    import ctypes
     import torch
+    # the copies bundled with torch match the HIP runtime torch has loaded
+    tlib = os.path.join(os.path.dirname(torch.__file__), "lib")
+    rb = ctypes.CDLL(os.path.join(tlib, "librocblas.so"))
+    rs = ctypes.CDLL(os.path.join(tlib, "librocsolver.so"))
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong
+    rb.rocblas_create_handle.argtypes = [ctypes.POINTER(vp)]
+    rb.rocblas_set_stream.argtypes = [vp, vp]
+    rb.rocblas_destroy_handle.argtypes = [vp]
+    rs.rocsolver_zgetrf_strided_batched.argtypes = [vp, i32, i32, vp, i32, i64, vp, i64, vp, i32]
+    rs.rocsolver_zgetrs_strided_batched.argtypes = [vp, i32, i32, i32, vp, i32, i64, vp, i64, vp,
+                                                    i32, i64, i32]
     dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    handle = vp()
+    if rb.rocblas_create_handle(ctypes.byref(handle)) != 0:
+        raise RuntimeError("rocblas_create_handle failed")
+    rb.rocblas_set_stream(handle, vp(stream.cuda_stream))
     rom = case.rom
     K = torch.tensor(rom.K, dtype=torch.complex128, device=dev)
     M = torch.tensor(rom.M, dtype=torch.complex128, device=dev)
     D = torch.tensor(rom.D, dtype=torch.complex128, device=dev)
     w = torch.tensor(2 * np.pi * case.freq, dtype=torch.float64, device=dev)
-    F = torch.tensor(case.F, dtype=torch.complex128, device=dev)
-    chunk = 256
+    chunk = 512
+    # column-major A: build A^T in row-major torch storage (A is symmetric here, but
+    # transpose anyway so the layout is right for any operator)
+    Kt, Mt, Dt = K.T.contiguous(), M.T.contiguous(), D.T.contiguous()
+    A = torch.empty((chunk, N, N), dtype=torch.complex128, device=dev)
+    B = torch.empty((chunk, NRHS, N), dtype=torch.complex128, device=dev)
+    Ft = torch.tensor(np.ascontiguousarray(case.F.T), dtype=torch.complex128, device=dev)
+    ipiv = torch.empty((chunk, N), dtype=torch.int32, device=dev)
+    info = torch.empty(chunk, dtype=torch.int32, device=dev)
+    status = []
 
     def run():
-        for s in range(0, NFREQ, chunk):
-            ww = w[s:s + chunk].to(torch.complex128)[:, None, None]
-            A = (1 + 1j * rom.eta) * K + 1j * ww * D - ww * ww * M
-            LU, piv = torch.linalg.lu_factor(A)
-            torch.linalg.lu_solve(LU, piv, F.expand(A.shape[0], -1, -1))
+        for s0 in range(0, NFREQ, chunk):
+            ww = w[s0:s0 + chunk].to(torch.complex128)[:, None, None]
+            torch.add((1 + 1j * rom.eta) * Kt, 1j * ww * Dt - ww * ww * Mt, out=A)
+            B.copy_(Ft.expand(chunk, -1, -1))
+            status.append(rs.rocsolver_zgetrf_strided_batched(handle, N, N, vp(A.data_ptr()), N, N * N,
+                                                              vp(ipiv.data_ptr()), N,
+                                                              vp(info.data_ptr()), chunk))
+            status.append(rs.rocsolver_zgetrs_strided_batched(handle, 111, N, NRHS, vp(A.data_ptr()), N,
+                                                              N * N, vp(ipiv.data_ptr()), N,
+                                                              vp(B.data_ptr()), N, N * NRHS, chunk))
 
-    stream = torch.cuda.current_stream()
     wall, _ = time_gpu(run, steps, warmup, stream)
-    return NFREQ * steps / wall
+    # correctness of the vendor result on the last chunk
+    X = B.cpu().numpy().transpose(0, 2, 1)
+    Xc = synth_closed_last_chunk(case, chunk)
+    err = float((np.abs(X - Xc).max(axis=(1, 2)) / np.abs(Xc).max(axis=(1, 2))).max())
+    rb.rocblas_destroy_handle(handle)
+    if any(st != 0 for st in status):
+        raise RuntimeError(f"rocsolver status {set(status)}")
+    return NFREQ * steps / wall, err
+
+
+def synth_closed_last_chunk(case, chunk):
+    from staccato_amd import synth
+    return synth.closed_form(case.rom, case.freq[NFREQ - chunk:], case.F)
 
 
 def cpu_baseline(case, budget_s=20.0):
@@ -278,6 +328,8 @@ def main():
     r = res[main_mode]
     peak = PEAK_FP32_TFLOPS if main_mode == "refined" else PEAK_FP64_TFLOPS
     achieved = W * NFREQ / (r["kernel_ms"] * 1e-3) / 1e12
+    fk, wk = TRAFFIC_KIB[main_mode]
+    traffic = (fk * FETCH_SCALE + wk * WRITE_SCALE) * 1024.0
     out = {
         "metric": "freq points solved/s at n=256 vs plain FP64 GPU LU (rel. residual <=1e-10)",
         "value": r["pts_per_s"],
@@ -296,7 +348,8 @@ def main():
                    "n": N, "nfreq": NFREQ, "nrhs": NRHS, "mode": main_mode,
                    "tol": TOL_REFINED if main_mode == "refined" else TOL_PLAIN},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic,
+                     "traffic_unit": "bytes per launch (HBM, rocprof FETCH_SIZE/WRITE_SIZE)",
                      "work_per_point_flop": W, "points_per_launch": NFREQ},
         "results": res,
     }
@@ -304,9 +357,11 @@ def main():
         out["speedup_vs_plain_fp64"] = res["refined"]["pts_per_s"] / res["plain"]["pts_per_s"]
     if not args.no_vendor:
         try:
-            v = vendor_fp64(case, max(1, args.steps // 10), 1)
+            v, verr = vendor_fp64(case, max(1, args.steps // 5), 1)
             out["vendor_fp64_pts_per_s"] = v
-        except Exception as e:  # vendor stack missing a kernel: report, do not fail
+            out["vendor_fp64"] = "rocsolver_zgetrf/zgetrs_strided_batched on materialised A(w)"
+            out["vendor_fp64_rel_error_vs_closed_form"] = verr
+        except Exception as e:  # vendor stack unavailable: report, do not fail
             out["vendor_fp64_pts_per_s"] = None
             out["vendor_error"] = repr(e)[:200]
     if not args.no_cpu:
