@@ -110,7 +110,7 @@ int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int jb_cap, int* ld
         // a thin panel starves the trailing update: take the whole CU instead
         for (const SysDesc& s : sys) {
             Layout L = plan_layout(s.n, s.nrhs, ts, kBudgetSmall, jb_cap, adj);
-            if (L.lds > kBudgetSmall || (!L.small && L.jb < 16)) budget = kBudgetLarge;
+            if (L.lds > kBudgetSmall || (!L.small && L.jb < 12)) budget = kBudgetLarge;
         }
     }
     int lds = 0;

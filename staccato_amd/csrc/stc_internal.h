@@ -88,8 +88,8 @@ struct Layout {
 
 __host__ __device__ inline int align16(long long b) { return (int)((b + 15) & ~15LL); }
 
-constexpr int kChunkCols = 32;    // trailing columns per chunk (kernel: kChunk)
-constexpr int kChunkLdCols = 34;  // padded row length of the U12 chunk in LDS
+constexpr int kUwCols = 16;       // trailing columns per wave chunk (kernel: kUw)
+constexpr int kUwLdCols = 17;     // padded row length of a wave's U12 chunk in LDS
 constexpr int kSolveBlk = 32;     // diagonal block of the blocked solves
 
 // Panel columns each wavefront keeps in registers (x MAXR rows per lane):
@@ -132,16 +132,17 @@ inline Layout plan_layout(int n, int nrhs, int ts, int budget, int jb_cap, int a
             return L;
         }
     }
-    // blocked: LDS panel (n x jb) + one U12 chunk (jb x kChunkLdCols); the
-    // solve phase reuses the same region for its vectors + a staged diagonal block
+    // blocked: LDS panel (n x jb) + one U12 chunk per wave (jb x kUwLdCols) +
+    // the inverted diagonal block (jb x jb); the solve phase reuses the same
+    // region for its vectors + a staged diagonal block
     const int jbs[] = {32, 28, 24, 20, 16, 12, 8, 4};
     for (int a = 0; a < 8; ++a) {
         int jb = jbs[a];
         if (jb > jb_cap) continue;
-        int need = fixed + align16((long long)n * jb * ts) + align16((long long)jb * kChunkLdCols * ts) +
+        int need = fixed + align16((long long)n * jb * ts) + align16((long long)kWaves * jb * kUwLdCols * ts) +
                    align16((long long)jb * jb * ts);
         if (need <= budget) {
-            L.jb = jb; L.cw = kChunkCols; L.lds = need;
+            L.jb = jb; L.cw = kUwCols; L.lds = need;
             for (int G = Gmax; G >= 1; --G) {
                 int ns = fixed + lds_solve_bytes(n, G, ts, adj) +
                          align16((long long)kSolveBlk * kSolveBlk * ts);

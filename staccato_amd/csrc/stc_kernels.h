@@ -33,6 +33,43 @@ struct cx {
 };
 using z64 = cx<double>;
 
+// ---- explicit address spaces ---------------------------------------------------------------
+// Pointers reach device code through the system descriptor as generic
+// pointers, which the compiler turns into flat instructions with 64-bit
+// address arithmetic.  These accessors state the address space: 1 = global
+// memory (global_load/store, scalar base + 32-bit offset), 3 = LDS (ds_*).
+template <typename T>
+struct vec2;
+template <>
+struct vec2<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct vec2<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <int AS, typename T>
+__device__ __forceinline__ cx<T> ldc(const cx<T>* p) {
+    using V = typename vec2<T>::type;
+    const V v = *(const __attribute__((address_space(AS))) V*)(p);
+    cx<T> c;
+    c.re = v.x;
+    c.im = v.y;
+    return c;
+}
+template <int AS, typename T>
+__device__ __forceinline__ void stc(cx<T>* p, cx<T> a) {
+    using V = typename vec2<T>::type;
+    V v;
+    v.x = a.re;
+    v.y = a.im;
+    *(__attribute__((address_space(AS))) V*)(p) = v;
+}
+template <int AS, typename T>
+__device__ __forceinline__ T lds1(const T* p) {
+    return *(const __attribute__((address_space(AS))) T*)(p);
+}
+
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 template <typename T>
@@ -122,17 +159,17 @@ __device__ __forceinline__ double zabs(z64 a) { return hypot(a.re, a.im); }
 // ---- assembly: element e (column-major, ld = n) of A(w) in FP64 ---------------
 __device__ __forceinline__ z64 assemble_at(const SysDesc& s, size_t e, double w, double w2) {
     if (s.real_ops) {
-        const double k = s.K[e];
-        const double m = s.M[e];
-        const double d = s.has_D ? s.D[e] : 0.0;
+        const double k = lds1<1>(s.K + e);
+        const double m = lds1<1>(s.M + e);
+        const double d = s.has_D ? lds1<1>(s.D + e) : 0.0;
         return cmk<double>(k - w2 * m, s.eta * k + w * d);
     }
-    const double2 k = reinterpret_cast<const double2*>(s.K)[e];
-    const double2 m = reinterpret_cast<const double2*>(s.M)[e];
-    double2 d = make_double2(0.0, 0.0);
-    if (s.has_D) d = reinterpret_cast<const double2*>(s.D)[e];
-    return cmk<double>(k.x - s.eta * k.y - w2 * m.x - w * d.y,
-                       k.y + s.eta * k.x - w2 * m.y + w * d.x);
+    const z64 k = ldc<1>(reinterpret_cast<const z64*>(s.K) + e);
+    const z64 m = ldc<1>(reinterpret_cast<const z64*>(s.M) + e);
+    z64 d = czero<double>();
+    if (s.has_D) d = ldc<1>(reinterpret_cast<const z64*>(s.D) + e);
+    return cmk<double>(k.re - s.eta * k.im - w2 * m.re - w * d.im,
+                       k.im + s.eta * k.re - w2 * m.im + w * d.re);
 }
 
 // ---- pivot search: first index of max |re|+|im| over rows [start, m) -------------
@@ -269,169 +306,9 @@ struct Mfma<double> {
     }
 };
 
-constexpr int kChunk = 32;      // trailing columns per TRSM/update chunk
-constexpr int kChunkLd = kChunk + 2;
 constexpr int kSolveBlock = 32; // diagonal block of the blocked triangular solves
-
-// ---- U12 chunk = L11^{-1} A12 (wave substitution) -----------------------------------------
-// A12: logical rows [j0, j0+jb) of global columns [cb, cb+cw) (row r lives at
-// physical row rmap[r]); L11 = unit-lower P[0:jb, 0:jb] (LDS, ld ldp).  Each
-// wave solves columns wave, wave+4, ... (lanes = rows, jb <= 64) and the result
-// goes to Uc (LDS, row-major, ld kChunkLd) and back to S.
-template <typename T>
-__device__ void trsm_chunk(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb,
-                           const cx<T>* P, int ldp, cx<T>* Uc, int cb, int cw) {
-    constexpr int CPW = kChunk / kWaves;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const size_t prow = lane < jb ? (size_t)rmap[j0 + lane] : 0;
-    cx<T> y[CPW];
-#pragma unroll
-    for (int q = 0; q < CPW; ++q) {
-        const int c = wave + kWaves * q;
-        y[q] = (lane < jb && c < cw) ? S[prow + (size_t)(cb + c) * ld] : czero<T>();
-    }
-    for (int k = 0; k < jb - 1; ++k) {
-        const cx<T> lk = (lane > k && lane < jb) ? P[lane + (size_t)k * ldp] : czero<T>();
-#pragma unroll
-        for (int q = 0; q < CPW; ++q) {
-            const cx<T> vk = cbcast(y[q], k);
-            cfms(y[q], lk, vk);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < CPW; ++q) {
-        const int c = wave + kWaves * q;
-        if (lane < jb && c < cw) {
-            Uc[lane * kChunkLd + c] = y[q];
-            S[prow + (size_t)(cb + c) * ld] = y[q];
-        }
-    }
-}
-
-// ---- trailing update of one chunk on MFMA -------------------------------------------------
-// C = logical rows [j0+jb, n) x columns [cb, cb+cw) of S (physical rows via rmap)
-//   -= L21 U12,  L21 = P[jb:m, 0:jb] (LDS), U12 = Uc (jb x cw, LDS, row-major).
-// Computed transposed (C^T -= U^T L21^T) so that a lane's accumulators are 16
-// consecutive rows of one column: wave tile 32 rows x 32 columns (2 x 2 MFMA
-// tiles, real and imaginary accumulators, 4 real MFMAs per complex product).
-template <typename T>
-__device__ void trailing_mfma(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb, int m,
-                              const cx<T>* P, int ldp, const cx<T>* Uc, int cb, int cw) {
-    using M = Mfma<T>;
-    using acc_t = typename M::acc;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, lg = lane >> 4;
-    const int m2 = m - jb;
-    for (int r0 = wave * 32; r0 < m2; r0 += kWaves * 32) {
-        acc_t cre[2][2], cim[2][2];
-        size_t prow[2];
-        bool rok[2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const int row = r0 + 16 * a + li;
-            rok[a] = row < m2;
-            prow[a] = rok[a] ? (size_t)rmap[j0 + jb + row] : 0;
-        }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int col = 16 * b + M::row(lg, r);
-                    cx<T> v = czero<T>();
-                    if (rok[a] && col < cw) v = S[prow[a] + (size_t)(cb + col) * ld];
-                    cre[a][b][r] = v.re;
-                    cim[a][b][r] = v.im;
-                }
-#pragma unroll 2
-        for (int k0 = 0; k0 < jb; k0 += 4) {
-            const int k = k0 + lg;
-            const bool kok = k < jb;
-            cx<T> lv[2], uv[2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const int row = min(r0 + 16 * a + li, m2 - 1);
-                lv[a] = kok ? P[(jb + row) + (size_t)k * ldp] : czero<T>();
-            }
-#pragma unroll
-            for (int b = 0; b < 2; ++b) uv[b] = kok ? Uc[k * kChunkLd + 16 * b + li] : czero<T>();
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    cre[a][b] = M::run(-uv[b].re, lv[a].re, cre[a][b]);
-                    cre[a][b] = M::run(uv[b].im, lv[a].im, cre[a][b]);
-                    cim[a][b] = M::run(-uv[b].re, lv[a].im, cim[a][b]);
-                    cim[a][b] = M::run(-uv[b].im, lv[a].re, cim[a][b]);
-                }
-        }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int col = 16 * b + M::row(lg, r);
-                    if (rok[a] && col < cw)
-                        S[prow[a] + (size_t)(cb + col) * ld] = cmk<T>(cre[a][b][r], cim[a][b][r]);
-                }
-    }
-}
-
-// ---- trailing update of one chunk on FP64 FMA (VALU) ----------------------------------------
-// Same wave tile and lane ownership as trailing_mfma (lane: rows r0+16a+li,
-// columns 16b+4lg+r); on MI355X FP64 FMA and FP64 MFMA share one peak rate.
-template <typename T>
-__device__ void trailing_valu(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb, int m,
-                              const cx<T>* P, int ldp, const cx<T>* Uc, int cb, int cw) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, lg = lane >> 4;
-    const int m2 = m - jb;
-    for (int r0 = wave * 32; r0 < m2; r0 += kWaves * 32) {
-        cx<T> c[2][2][4];
-        size_t prow[2];
-        bool rok[2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const int row = r0 + 16 * a + li;
-            rok[a] = row < m2;
-            prow[a] = rok[a] ? (size_t)rmap[j0 + jb + row] : 0;
-        }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int col = 16 * b + 4 * lg + r;
-                    c[a][b][r] = (rok[a] && col < cw) ? S[prow[a] + (size_t)(cb + col) * ld]
-                                                      : czero<T>();
-                }
-        for (int k = 0; k < jb; ++k) {
-            cx<T> lv[2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a) lv[a] = P[(jb + min(r0 + 16 * a + li, m2 - 1)) + (size_t)k * ldp];
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const cx<T> u = Uc[k * kChunkLd + 16 * b + 4 * lg + r];
-#pragma unroll
-                    for (int a = 0; a < 2; ++a) cfms(c[a][b][r], lv[a], u);
-                }
-        }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int col = 16 * b + 4 * lg + r;
-                    if (rok[a] && col < cw) S[prow[a] + (size_t)(cb + col) * ld] = c[a][b][r];
-                }
-    }
-}
+constexpr int kUw = kUwCols;    // columns of one wave's U12 chunk
+constexpr int kUwLd = kUwLdCols;
 
 // ---- LDS carve-up ---------------------------------------------------------------------
 struct Lds {
@@ -459,34 +336,83 @@ __device__ __forceinline__ Lds carve(char* smem, int n) {
 // ---- register-resident panel factorisation (blocked path) --------------------------------
 // Wave w owns panel columns w, w+4, ... (CPW of them) and lane l owns local
 // rows l + 64 t (t < MAXR) of those columns, so the rank-1 updates and the row
-// interchanges never touch memory.  Per column jj: the owning wave searches
-// the pivot (first row of largest |re|+|im|), swaps and scales its column and
-// publishes the multipliers in lcol (LDS); after one barrier every wave swaps
-// rows jj <-> p in its other columns (lane shuffles) and applies the rank-1
-// update.  The factored panel ends in P (LDS, m x jb, ld m) and in S through
-// the final row map.
-template <typename T, int MAXR>
-__device__ __forceinline__ cx<T> reg_get(const cx<T> (&col)[MAXR], int row) {
-    const int slot = row >> 6;
-    cx<T> v = col[0];
-#pragma unroll
-    for (int t = 1; t < MAXR; ++t)
-        if (slot == t) v = col[t];
-    return cshfl(v, row & 63);
+// interchanges never touch memory.  The factored panel ends in P (LDS, m x jb,
+// ld m, LAPACK row order) and in S through the final row map.
+// ---- DPP lane permutations inside rows of 16 lanes (gfx9 encodings) ----------------------
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // lane i <-> 7-i within 8
+constexpr int kDppMirror = 0x140;     // lane i <-> 15-i within 16
+
+template <int CTRL>
+__device__ __forceinline__ int dpp(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false); }
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) { return __int_as_float(dpp<CTRL>(__float_as_int(v))); }
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp<CTRL>((int)b), hi = (unsigned)dpp<CTRL>((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform(double v) { return lane_read(v, 0); }
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Pivot search reduction over one wave: keeps (best, bpos, bg) of the lane
+// with the largest best, ties to the smallest bpos (the first row in LAPACK
+// order).  Four DPP steps reduce each row of 16 lanes, two lane shuffles
+// join the rows; the result is read from lane 0, so it is wave-uniform.
+template <typename T>
+__device__ __forceinline__ void wave_argmax3(T& best, int& bpos, int& bg) {
+    auto comb = [&](T ob, int op, int og) {
+        if (ob > best || (ob == best && op < bpos)) {
+            best = ob;
+            bpos = op;
+            bg = og;
+        }
+    };
+    comb(dpp<kDppXor1>(best), dpp<kDppXor1>(bpos), dpp<kDppXor1>(bg));
+    comb(dpp<kDppXor2>(best), dpp<kDppXor2>(bpos), dpp<kDppXor2>(bg));
+    comb(dpp<kDppHalfMirror>(best), dpp<kDppHalfMirror>(bpos), dpp<kDppHalfMirror>(bg));
+    comb(dpp<kDppMirror>(best), dpp<kDppMirror>(bpos), dpp<kDppMirror>(bg));
+    comb(__shfl_xor(best, 16, 64), __shfl_xor(bpos, 16, 64), __shfl_xor(bg, 16, 64));
+    comb(__shfl_xor(best, 32, 64), __shfl_xor(bpos, 32, 64), __shfl_xor(bg, 32, 64));
+    best = uniform(best);
+    bpos = uniform(bpos);
+    bg = uniform(bg);
 }
 
 // Register panel without row interchanges.  Rows stay where they were
 // gathered (gather row g = lane + 64 t); every lane tracks lpos[t], the LAPACK
 // logical position of its rows (a pivot swaps the positions jj <-> p only).
-// Pivot candidates at step jj are the rows with lpos >= jj; ties go to the
-// smallest lpos (first index, as izamax).  At the end the panel is scattered
-// once into logical order (P, LDS) and into S at the rows' unchanged physical
-// addresses, and rmap is rewritten: rmap[j0 + lpos] = physical row.
+// Column jj belongs to wave jj % 4.  A wave keeps its columns in x[] rotated
+// so that its next unfactored column is always x[0] (one rotation after each
+// of its columns, undone at the end), which reaches the pivot column without
+// a register select.  The owner publishes the multipliers (lcol) and the
+// pivot (sc) in one of two LDS buffers, alternating by column: one barrier
+// per column suffices, since a buffer is rewritten only two columns later,
+// after every wave has passed the barrier of the column in between.  Each
+// wave then reads the pivot row's entries of its own columns with v_readlane
+// (the pivot lane is uniform) and applies the rank-1 update.
+template <typename T, int MAXR, int CPW>
+__device__ __forceinline__ void rotate_cols(cx<T> (&x)[CPW][MAXR]) {
+#pragma unroll
+    for (int t = 0; t < MAXR; ++t) {
+        const cx<T> x0 = x[0][t];
+#pragma unroll
+        for (int q = 0; q + 1 < CPW; ++q) x[q][t] = x[q + 1][t];
+        x[CPW - 1][t] = x0;
+    }
+}
+
 template <typename T, int MAXR, int CPW>
 __device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, int jb, int* ipiv,
-                                 int* rmap, int* sc, cx<T>* lcol, cx<T>* P) {
+                                 int* rmap, int* sc, cx<T>* lcolbuf, cx<T>* P) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = n - j0;
+    constexpr int kLcolStride = 64 * MAXR;  // second buffer: lcolbuf + kLcolStride (<= n jb / 2)
     cx<T> x[CPW][MAXR];
     int lpos[MAXR];
     int prow[MAXR];
@@ -494,33 +420,28 @@ __device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, i
     for (int t = 0; t < MAXR; ++t) {
         const int r = lane + 64 * t;
         lpos[t] = r < m ? r : 0x7fffffff;
-        prow[t] = r < m ? rmap[j0 + r] : 0;
+        prow[t] = r < m ? lds1<3>(rmap + j0 + r) : 0;
     }
 #pragma unroll
     for (int q = 0; q < CPW; ++q)
 #pragma unroll
         for (int t = 0; t < MAXR; ++t) {
             const int c = wave + kWaves * q, r = lane + 64 * t;
-            x[q][t] = (c < jb && r < m) ? S[(size_t)prow[t] + (size_t)(j0 + c) * ld] : czero<T>();
+            x[q][t] = (c < jb && r < m) ? ldc<1>(S + (prow[t] + (j0 + c) * ld)) : czero<T>();
         }
+    int done = 0;  // columns of this wave already factored (= rotations applied)
     for (int jj = 0; jj < jb; ++jj) {
-        const int ow = jj & (kWaves - 1), oq = jj / kWaves;
-        if (wave == ow) {
-            cx<T> col[MAXR];
-#pragma unroll
-            for (int t = 0; t < MAXR; ++t) {
-                col[t] = x[0][t];
-#pragma unroll
-                for (int q = 1; q < CPW; ++q)
-                    if (q == oq) col[t] = x[q][t];
-            }
-            // argmax of |re|+|im| over rows with lpos >= jj; ties: smallest lpos
+        const int lb = (jj & 1) * kLcolStride, sb = 4 + 4 * (jj & 1);
+        if (wave == (jj & (kWaves - 1))) {
+            // argmax of |re|+|im| over rows with lpos >= jj; ties: smallest lpos.
+            // All-zero column: the smallest eligible lpos is jj itself, so the
+            // row stays (LAPACK: ipiv = jj) and no scaling happens.
             T best = T(-1);
             int bpos = 0x7fffffff, bg = 0;
 #pragma unroll
             for (int t = 0; t < MAXR; ++t) {
                 if (lpos[t] >= jj && lpos[t] != 0x7fffffff) {
-                    const T v = cabs1(col[t]);
+                    const T v = cabs1(x[0][t]);
                     if (v > best || (v == best && lpos[t] < bpos)) {
                         best = v;
                         bpos = lpos[t];
@@ -528,52 +449,32 @@ __device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, i
                     }
                 }
             }
+            wave_argmax3(best, bpos, bg);
+            const int ps = bg >> 6, pln = bg & 63;
+            cx<T> pv = x[0][0];
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const T ob = __shfl_xor(best, off, 64);
-                const int op = __shfl_xor(bpos, off, 64);
-                const int og = __shfl_xor(bg, off, 64);
-                if (ob > best || (ob == best && op < bpos)) { best = ob; bpos = op; bg = og; }
-            }
-            const cx<T> pv = reg_get<T, MAXR>(col, bg);
+            for (int t = 1; t < MAXR; ++t)
+                if (ps == t) pv = x[0][t];
+            pv = cshfl(pv, pln);
             const bool zero = pv.re == T(0) && pv.im == T(0);
-            int pl = bpos, pg = bg;
-            if (zero) {
-                // no pivot: the row at logical position jj stays (LAPACK: ipiv = jj)
-                pl = jj;
-                int g = 0x7fffffff;
-#pragma unroll
-                for (int t = 0; t < MAXR; ++t)
-                    if (lpos[t] == jj) g = lane + 64 * t;
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) g = min(g, __shfl_xor(g, off, 64));
-                pg = g;
-            } else {
-                const cx<T> rp = crecip(pv);
-#pragma unroll
-                for (int t = 0; t < MAXR; ++t) {
-                    // rows below the pivot after this step: lpos > jj, except the pivot row
-                    const bool below = lpos[t] >= jj && lpos[t] != 0x7fffffff && (lane + 64 * t) != pg;
-                    if (below) col[t] = cmul(col[t], rp);
-                }
-            }
+            const cx<T> rp = zero ? czero<T>() : crecip(pv);
 #pragma unroll
             for (int t = 0; t < MAXR; ++t) {
-                const bool below = lpos[t] >= jj && lpos[t] != 0x7fffffff && (lane + 64 * t) != pg;
-                lcol[lane + 64 * t] = below ? col[t] : czero<T>();
-#pragma unroll
-                for (int q = 0; q < CPW; ++q)
-                    if (q == oq) x[q][t] = col[t];
+                // rows below the pivot after this step: lpos >= jj, except the pivot row
+                // (slot ps of lane pln; both uniform)
+                const bool below = lpos[t] >= jj && lpos[t] != 0x7fffffff && (t != ps || lane != pln);
+                if (below && !zero) x[0][t] = cmul(x[0][t], rp);
+                stc<3>(lcolbuf + lb + lane + 64 * t, below ? x[0][t] : czero<T>());
             }
             if (lane == 0) {
-                sc[0] = pg;
-                sc[2] = pl;
-                ipiv[j0 + jj] = j0 + pl;
+                sc[sb] = bg;
+                sc[sb + 2] = bpos;
+                ipiv[j0 + jj] = j0 + bpos;
                 if (zero && sc[1] == 0) sc[1] = j0 + jj + 1;
             }
         }
         __syncthreads();
-        const int pg = sc[0], pl = sc[2];
+        const int pg = uniform(sc[sb]), pl = uniform(sc[sb + 2]);
         // logical positions: jj <-> pl
 #pragma unroll
         for (int t = 0; t < MAXR; ++t) {
@@ -582,18 +483,30 @@ __device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, i
         }
         cx<T> l[MAXR];
 #pragma unroll
-        for (int t = 0; t < MAXR; ++t) l[t] = lcol[lane + 64 * t];
+        for (int t = 0; t < MAXR; ++t) l[t] = ldc<3>(lcolbuf + lb + lane + 64 * t);
+        const int ps = pg >> 6, pln = pg & 63;
 #pragma unroll
         for (int q = 0; q < CPW; ++q) {
-            const int c = wave + kWaves * q;
+            const int c = wave + kWaves * (q + done);  // x[q] holds this column while unfactored
             if (c > jj && c < jb) {
-                const cx<T> u = reg_get<T, MAXR>(x[q], pg);
+                cx<T> v = x[q][0];
+#pragma unroll
+                for (int t = 1; t < MAXR; ++t)
+                    if (ps == t) v = x[q][t];
+                const cx<T> u = cbcast(v, pln);
 #pragma unroll
                 for (int t = 0; t < MAXR; ++t) cfms(x[q][t], l[t], u);
             }
         }
-        __syncthreads();
+        if (wave == (jj & (kWaves - 1))) {
+            rotate_cols(x);
+            ++done;
+        }
     }
+    for (int r = done; r < CPW; ++r) rotate_cols(x);  // back to x[q] = column wave + 4 q
+    // every wave has read the last multipliers before P (which holds the
+    // lcol buffers) is overwritten
+    __syncthreads();
 #pragma unroll
     for (int t = 0; t < MAXR; ++t) {
         const int r = lane + 64 * t;
@@ -605,8 +518,8 @@ __device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, i
         for (int t = 0; t < MAXR; ++t) {
             const int c = wave + kWaves * q, r = lane + 64 * t;
             if (c < jb && r < m) {
-                P[lpos[t] + (size_t)c * m] = x[q][t];
-                S[(size_t)prow[t] + (size_t)(j0 + c) * ld] = x[q][t];
+                stc<3>(P + (lpos[t] + c * m), x[q][t]);
+                stc<1>(S + (prow[t] + (j0 + c) * ld), x[q][t]);
             }
         }
 }
@@ -652,49 +565,160 @@ __device__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<T>* Linv_lds, cx
     }
 }
 
-// ---- U12 chunk = Linv A12 (parallel product) --------------------------------------------------
-// Thread t: column c = t % 32 of the chunk, rows t/32 + 8 s.  A12 comes
-// straight from S (logical rows j0.. via rmap), all jb loads in flight.  The
-// result goes to Uc only: the threads of one column read every row of A12,
-// so S is updated from Uc after the next barrier (uc_to_slot).
-template <typename T, int JB8>
-__device__ void trsm_linv(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb,
-                          const cx<T>* Linv, cx<T>* Uc, int cb, int cw) {
-    const int tid = threadIdx.x, c = tid & 31, i0 = tid >> 5;
-    cx<T> acc[JB8];
+// ---- block row and trailing update after one panel, wave-parallel ---------------------
+// The trailing columns [j0+jb, n) are cut into chunks of kUw columns; wave w
+// takes chunks w, w+4, ... and needs no block barrier:
+//   U12 = Linv A12 on MFMA (A12 = logical rows [j0, j0+jb) of the chunk, all
+//   of its loads in flight together), written to S and to the wave's LDS
+//   buffer Uw (jb x kUw, row-major, ld kUwLd);
+//   C -= L21 U12 for logical rows [j0+jb, n) in tiles of 32 rows, the next
+//   tile's C loaded before the current tile's products.
+// Both products are computed transposed so that the MFMA operands come
+// straight from the column-major slot and LDS panel: D = A12^T Linv^T = U12^T
+// and D = U12^T L21^T = (L21 U12)^T.  Complex products take 4 real MFMAs.
+// jb <= 32.
+template <typename T>
+__device__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb, int n,
+                             const cx<T>* P, int ldp, const cx<T>* Linv, cx<T>* Uw_all) {
+    using M = Mfma<T>;
+    using acc_t = typename M::acc;
+    constexpr int KS = 8;  // k-steps of 4 for jb <= 32
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int c0 = j0 + jb, m2 = n - c0;
+    const int nch = (m2 + kUw - 1) / kUw;
+    cx<T>* Uw = Uw_all + wave * jb * kUwLd;
+    auto load_a12 = [&](int cb, int cw, cx<T> (&av)[KS]) {
 #pragma unroll
-    for (int q = 0; q < JB8; ++q) acc[q] = czero<T>();
-    if (c < cw) {
-        const size_t col = (size_t)(cb + c) * ld;
-        for (int k0 = 0; k0 < jb; k0 += 8) {
-            cx<T> a[8];
+        for (int s = 0; s < KS; ++s) {
+            const int k = 4 * s + lg;
+            av[s] = (k < jb && li < cw) ? ldc<1>(S + (lds1<3>(rmap + j0 + k) + (cb + li) * ld))
+                                        : czero<T>();
+        }
+    };
+    auto load_c = [&](int r0, int cb, int cw, acc_t (&re)[2], acc_t (&im)[2], int (&pr)[2]) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                a[u] = (k0 + u < jb) ? S[(size_t)rmap[j0 + k0 + u] + col] : czero<T>();
+        for (int a = 0; a < 2; ++a) {
+            const int row = r0 + 16 * a + li;
+            const bool ok = row < m2;
+            pr[a] = ok ? lds1<3>(rmap + c0 + row) : 0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int k = k0 + u;
+            for (int r = 0; r < 4; ++r) {
+                const int col = M::row(lg, r);
+                cx<T> v = czero<T>();
+                if (ok && col < cw) v = ldc<1>(S + (pr[a] + (cb + col) * ld));
+                re[a][r] = v.re;
+                im[a][r] = v.im;
+            }
+        }
+    };
+    // complex64 keeps two C tiles and the next chunk's A12 in flight; the
+    // complex128 registers allow one tile ahead
+    constexpr bool kDeep = sizeof(T) == 4;
+    cx<T> av[KS];
+    if (kDeep && wave < nch) load_a12(c0 + wave * kUw, min(kUw, n - c0 - wave * kUw), av);
+    for (int ch = wave; ch < nch; ch += kWaves) {
+        const int cb = c0 + ch * kUw, cw = min(kUw, n - cb);
+        if (!kDeep) load_a12(cb, cw, av);
+        // ---- U12 = Linv A12 ----
+        {
+            acc_t ure[2], uim[2];
 #pragma unroll
-                for (int q = 0; q < JB8; ++q) {
-                    const int i = i0 + 8 * q;
-                    if (k < jb && i < jb && k <= i) cfma(acc[q], Linv[i + k * jb], a[u]);
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) ure[a][r] = uim[a][r] = T(0);
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                if (4 * s < jb) {
+                    const int k = 4 * s + lg;
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) {
+                        const int j = 16 * a + li;
+                        const cx<T> yv = (k < jb && j < jb) ? ldc<3>(Linv + j + k * jb) : czero<T>();
+                        ure[a] = M::run(av[s].re, yv.re, ure[a]);
+                        ure[a] = M::run(-av[s].im, yv.im, ure[a]);
+                        uim[a] = M::run(av[s].re, yv.im, uim[a]);
+                        uim[a] = M::run(av[s].im, yv.re, uim[a]);
+                    }
+                }
+            }
+            // lane holds U12[16a + li][M::row(lg, r)]
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int row = 16 * a + li;
+                if (row < jb) {
+                    const int prow = lds1<3>(rmap + j0 + row);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int col = M::row(lg, r);
+                        const cx<T> v = cmk<T>(ure[a][r], uim[a][r]);
+                        stc<3>(Uw + row * kUwLd + col, v);
+                        if (col < cw) stc<1>(S + (prow + (cb + col) * ld), v);
+                    }
                 }
             }
         }
+        __builtin_amdgcn_wave_barrier();
+        // ---- C -= L21 U12, tiles of 32 rows, loaded ahead of their products ----
+        acc_t cre[2], cim[2], bre[2], bim[2];
+        int prow[2], bprow[2];
+        load_c(0, cb, cw, cre, cim, prow);
+        if (kDeep && 32 < m2) load_c(32, cb, cw, bre, bim, bprow);
+        // the next chunk's A12 (its rows are final: the U12 of this chunk only
+        // touches this chunk's columns)
+        if (kDeep && ch + kWaves < nch) load_a12(cb + kWaves * kUw, min(kUw, n - cb - kWaves * kUw), av);
+        for (int r0 = 0; r0 < m2; r0 += 32) {
+            acc_t nre[2], nim[2];
+            int nprow[2];
+            const bool more2 = kDeep ? r0 + 64 < m2 : r0 + 32 < m2;
+            if (more2) load_c(kDeep ? r0 + 64 : r0 + 32, cb, cw, nre, nim, nprow);
 #pragma unroll
-        for (int q = 0; q < JB8; ++q) {
-            const int i = i0 + 8 * q;
-            if (i < jb) Uc[i * kChunkLd + c] = acc[q];
+            for (int s = 0; s < KS; ++s) {
+                if (4 * s < jb) {
+                    const int k = 4 * s + lg;
+                    const bool kok = k < jb;
+                    const cx<T> uv = kok ? ldc<3>(Uw + k * kUwLd + li) : czero<T>();
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) {
+                        const int row = min(r0 + 16 * a + li, m2 - 1);
+                        const cx<T> lv = kok ? ldc<3>(P + ((jb + row) + k * ldp)) : czero<T>();
+                        cre[a] = M::run(-uv.re, lv.re, cre[a]);
+                        cre[a] = M::run(uv.im, lv.im, cre[a]);
+                        cim[a] = M::run(-uv.re, lv.im, cim[a]);
+                        cim[a] = M::run(-uv.im, lv.re, cim[a]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int row = r0 + 16 * a + li;
+                if (row < m2) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int col = M::row(lg, r);
+                        if (col < cw) stc<1>(S + (prow[a] + (cb + col) * ld), cmk<T>(cre[a][r], cim[a][r]));
+                    }
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                if (kDeep) {
+                    cre[a] = bre[a];
+                    cim[a] = bim[a];
+                    prow[a] = bprow[a];
+                    if (more2) {
+                        bre[a] = nre[a];
+                        bim[a] = nim[a];
+                        bprow[a] = nprow[a];
+                    }
+                } else if (more2) {
+                    cre[a] = nre[a];
+                    cim[a] = nim[a];
+                    prow[a] = nprow[a];
+                }
+            }
         }
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void uc_to_slot(cx<T>* __restrict__ S, int ld, const int* rmap, int j0,
-                                           int jb, const cx<T>* Uc, int cb, int cw) {
-    for (int e = threadIdx.x; e < jb * kChunk; e += kThreads) {
-        const int i = e / kChunk, c = e % kChunk;
-        if (c < cw) S[(size_t)rmap[j0 + i] + (size_t)(cb + c) * ld] = Uc[i * kChunkLd + c];
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -718,9 +742,9 @@ __device__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* __r
 
     const int jbmax = lay.jb;
     cx<T>* P = reinterpret_cast<cx<T>*>(L.main);
-    cx<T>* Uc = reinterpret_cast<cx<T>*>(L.main + align16((long long)n * jbmax * sizeof(cx<T>)));
-    cx<T>* Linv = reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(Uc) +
-                                          align16((long long)jbmax * kChunkLd * sizeof(cx<T>)));
+    cx<T>* Uw = reinterpret_cast<cx<T>*>(L.main + align16((long long)n * jbmax * sizeof(cx<T>)));
+    cx<T>* Linv = reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(Uw) +
+                                          align16((long long)kWaves * jbmax * kUwLd * sizeof(cx<T>)));
     cx<T>* lcol = P;  // the multiplier column lives in the panel region while the panel is in registers
     const size_t nblk = (size_t)((n + jbmax - 1) / jbmax);
     cx<T>* Lg = S + nn;
@@ -733,14 +757,8 @@ __device__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* __r
         invert_diag<T, CPW>(P, m, jb, Linv, Lg + (size_t)blk * jbmax * jbmax,
                             Ug + (size_t)blk * jbmax * jbmax);
         __syncthreads();
-        for (int cb = j0 + jb; cb < n; cb += kChunk) {
-            const int cw = min(kChunk, n - cb);
-            trsm_linv<T, 8>(S, n, L.rmap, j0, jb, Linv, Uc, cb, cw);
-            __syncthreads();
-            uc_to_slot(S, n, L.rmap, j0, jb, Uc, cb, cw);
-            trailing_mfma(S, n, L.rmap, j0, jb, m, P, m, Uc, cb, cw);
-            __syncthreads();
-        }
+        block_update(S, n, L.rmap, j0, jb, n, P, m, Linv, Uw);
+        __syncthreads();
     }
 }
 
@@ -784,62 +802,90 @@ __device__ __forceinline__ const cx<T>* stage_diag(const TriCtx<T>& t, int k0, i
     return t.LU + k0 + (size_t)k0 * t.ld;
 }
 
-template <typename T, bool CONJ, bool TRANS>
-__device__ __forceinline__ void rect_update(const TriCtx<T>& t, int i, int k0, int kb, cx<T>* w,
-                                            int gc) {
+// Loads per round of the triangular-solve products: a whole diagonal block
+// (kb <= kSolveBlock) of complex64, half of one of complex128.
+template <typename T>
+struct SolveRound {
+    static constexpr int K = sizeof(T) == 4 ? 32 : 16;
+};
+
+template <int LUAS, typename T, bool CONJ, bool TRANS>
+__device__ __forceinline__ void rect_update_as(const TriCtx<T>& t, int i, int k0, int kb, cx<T>* w,
+                                               int gc) {
     // w[i] -= sum_{k<kb} op(LU)(i, k0+k) w[k0+k];  op = (TRANS ? element (k0+k, i) : (i, k0+k)),
-    // conjugated when CONJ
+    // conjugated when CONJ.  Every load of a round is issued before the first use.
+    // LUAS: address space of the factors (1 = global slot, 0 = generic).
+    constexpr int KR = SolveRound<T>::K;
+    k0 = __builtin_amdgcn_readfirstlane(k0);
+    kb = __builtin_amdgcn_readfirstlane(kb);
+    gc = __builtin_amdgcn_readfirstlane(gc);
+    const int n = __builtin_amdgcn_readfirstlane(t.n);
     cx<T> acc[kMaxGroup];
 #pragma unroll
     for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
     const size_t pi = t.row(i);
-    for (int kk = 0; kk < kb; kk += 8) {
-        cx<T> l[8];
+    for (int kk = 0; kk < kb; kk += KR) {
+        cx<T> l[KR];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int k = k0 + kk + u;
-            l[u] = czero<T>();
-            if (kk + u < kb) l[u] = TRANS ? t.LU[t.row(k) + (size_t)i * t.ld] : t.LU[pi + (size_t)k * t.ld];
-            if (CONJ) l[u] = cconj(l[u]);
+        for (int u = 0; u < KR; ++u) {
+            const int k = k0 + min(kk + u, kb - 1);
+            l[u] = ldc<LUAS>(TRANS ? t.LU + t.row(k) + (size_t)i * t.ld : t.LU + pi + (size_t)k * t.ld);
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (kk + u < kb)
+        for (int u = 0; u < KR; ++u) {
+            if (kk + u < kb) {
+                const cx<T> lu = CONJ ? cconj(l[u]) : l[u];
 #pragma unroll
                 for (int g = 0; g < kMaxGroup; ++g)
-                    if (g < gc) cfma(acc[g], l[u], w[k0 + kk + u + (size_t)g * t.n]);
+                    if (g < gc) cfma(acc[g], lu, ldc<3>(w + k0 + kk + u + (size_t)g * n));
+            }
+        }
     }
 #pragma unroll
     for (int g = 0; g < kMaxGroup; ++g)
         if (g < gc) {
-            cx<T> a = w[i + (size_t)g * t.n];
+            cx<T> a = ldc<3>(w + i + (size_t)g * n);
             a.re -= acc[g].re;
             a.im -= acc[g].im;
-            w[i + (size_t)g * t.n] = a;
+            stc<3>(w + i + (size_t)g * n, a);
         }
 }
 
-// y_b = Binv_b w_b for one diagonal block (lanes = rows, one wave per column).
+// Factors in the global slot (blocked path, rmap set) or in LDS (small path).
+template <typename T, bool CONJ, bool TRANS>
+__device__ __forceinline__ void rect_update(const TriCtx<T>& t, int i, int k0, int kb, cx<T>* w,
+                                            int gc) {
+    if (t.rmap) rect_update_as<1, T, CONJ, TRANS>(t, i, k0, kb, w, gc);
+    else rect_update_as<0, T, CONJ, TRANS>(t, i, k0, kb, w, gc);
+}
+
+// y_b = Binv_b w_b for one diagonal block (lanes = rows, one wave per column);
+// Binv_b in the global slot.
 template <typename T, bool LOWER>
 __device__ __forceinline__ void block_inv_apply(const cx<T>* __restrict__ Bi, int k0, int kb, cx<T>* w,
                                                 int n, int gc) {
+    constexpr int KR = SolveRound<T>::K;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    k0 = __builtin_amdgcn_readfirstlane(k0);
+    kb = __builtin_amdgcn_readfirstlane(kb);
+    gc = __builtin_amdgcn_readfirstlane(gc);
+    n = __builtin_amdgcn_readfirstlane(n);
+    const int row = min(lane, kb - 1);
     for (int g = wave; g < gc; g += kWaves) {
         cx<T>* y = w + (size_t)g * n + k0;
         cx<T> acc = czero<T>();
-        for (int k8 = 0; k8 < kb; k8 += 8) {
-            cx<T> bv[8];
+        for (int k8 = 0; k8 < kb; k8 += KR) {
+            cx<T> bv[KR];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < KR; ++u) bv[u] = ldc<1>(Bi + row + (size_t)min(k8 + u, kb - 1) * kb);
+#pragma unroll
+            for (int u = 0; u < KR; ++u) {
                 const int k = k8 + u;
-                const bool in = lane < kb && k < kb && (LOWER ? k <= lane : k >= lane);
-                bv[u] = in ? Bi[lane + (size_t)k * kb] : czero<T>();
+                const bool in = k < kb && (LOWER ? k <= lane : k >= lane);
+                if (k < kb) cfma(acc, in ? bv[u] : czero<T>(), ldc<3>(y + k));
             }
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (k8 + u < kb) cfma(acc, bv[u], y[k8 + u]);
         }
-        if (lane < kb) y[lane] = acc;
+        if (lane < kb) stc<3>(y + lane, acc);
     }
 }
 
@@ -1030,7 +1076,8 @@ template <bool ADJ>
 __device__ void residual(const SysDesc& s, double w, double w2, const double2* bg, int ldb,
                          const z64* bl, const z64* xs, z64* rs, int gc, bool need_anorm,
                          double* red, double (&out)[kRedSlots]) {
-    const int n = s.n, tid = threadIdx.x;
+    const int n = __builtin_amdgcn_readfirstlane(s.n), tid = threadIdx.x;
+    gc = __builtin_amdgcn_readfirstlane(gc);
     // pass 1: r = b - op(A) x (and the row sums of |op(A)|), 4 columns of the
     // operators in flight per round
     double amax = 0.0;
@@ -1038,7 +1085,8 @@ __device__ void residual(const SysDesc& s, double w, double w2, const double2* b
         z64 acc[kMaxGroup];
 #pragma unroll
         for (int g = 0; g < kMaxGroup; ++g)
-            acc[g] = g < gc ? (bg ? ld_z(bg + i + (size_t)g * ldb) : bl[i + (size_t)g * n])
+            acc[g] = g < gc ? (bg ? ldc<1>(reinterpret_cast<const z64*>(bg) + i + (size_t)g * ldb)
+                                  : ldc<3>(bl + i + (size_t)g * n))
                             : czero<double>();
         double asum = 0.0;
         for (int j0 = 0; j0 < n; j0 += 4) {
@@ -1051,8 +1099,8 @@ __device__ void residual(const SysDesc& s, double w, double w2, const double2* b
                 for (int u = 0; u < 4; ++u) {
                     const int j = min(j0 + u, n - 1);
                     const size_t e = ADJ ? (size_t)j + (size_t)i * n : (size_t)i + (size_t)j * n;
-                    const double k = Kp[e], mm = Mp[e];
-                    const double d = s.has_D ? Dp[e] : 0.0;
+                    const double k = lds1<1>(Kp + e), mm = lds1<1>(Mp + e);
+                    const double d = s.has_D ? lds1<1>(Dp + e) : 0.0;
                     a[u] = cmk<double>(k - w2 * mm, (ADJ ? -1.0 : 1.0) * (s.eta * k + w * d));
                 }
             } else {
@@ -1069,14 +1117,14 @@ __device__ void residual(const SysDesc& s, double w, double w2, const double2* b
                     if (need_anorm) asum += zabs(a[u]);
 #pragma unroll
                     for (int g = 0; g < kMaxGroup; ++g)
-                        if (g < gc) cfms(acc[g], a[u], xs[j0 + u + (size_t)g * n]);
+                        if (g < gc) cfms(acc[g], a[u], ldc<3>(xs + j0 + u + (size_t)g * n));
                 }
             }
         }
         amax = nmax(amax, asum);
 #pragma unroll
         for (int g = 0; g < kMaxGroup; ++g)
-            if (g < gc) rs[i + (size_t)g * n] = acc[g];
+            if (g < gc) stc<3>(rs + i + (size_t)g * n, acc[g]);
     }
     __syncthreads();
     // pass 2: the norms of the backward error
@@ -1314,7 +1362,8 @@ __device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, cha
         solvep = L.main + align16((long long)nn * sizeof(cx<T>));
     } else {
         if (kp.scratch == nullptr || kp.slot_elems < slot_elems_for(n, lay.jb) ||
-            lay.jb > kWaves * panel_cols_per_wave((int)sizeof(cx<T>), MAXR) || lay.jb > 64) {
+            lay.jb > kWaves * panel_cols_per_wave((int)sizeof(cx<T>), MAXR) || lay.jb > 32 ||
+            2 * 64 * MAXR > n * lay.jb) {
             // no factor slot, or a panel wider than the register panel: report, touch nothing
             if (tid == 0) s.info[fidx] = -2;
             __syncthreads();
