@@ -307,6 +307,7 @@ struct Mfma<double> {
 };
 
 constexpr int kSolveBlock = 32; // diagonal block of the blocked triangular solves
+constexpr int kResCols = 16;    // operator columns per load round of the residual
 constexpr int kUw = kUwCols;    // columns of one wave's U12 chunk
 constexpr int kUwLd = kUwLdCols;
 
@@ -734,8 +735,8 @@ __device__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* __r
     constexpr int CPW = panel_cols_per_wave((int)sizeof(cx<T>), MAXR);
     const int n = s.n, tid = threadIdx.x;
     const size_t nn = (size_t)n * n;
-#pragma unroll 4
-    for (size_t e = tid; e < nn; e += kThreads) S[e] = to_t<T>(assemble_at(s, e, w, w2));
+#pragma unroll 16
+    for (int e = tid; e < (int)nn; e += kThreads) stc<1>(S + e, to_t<T>(assemble_at(s, e, w, w2)));
     for (int r = tid; r < n; r += kThreads) L.rmap[r] = r;
     if (tid == 0) L.sc[1] = 0;
     __syncthreads();
@@ -1078,8 +1079,8 @@ __device__ void residual(const SysDesc& s, double w, double w2, const double2* b
                          double* red, double (&out)[kRedSlots]) {
     const int n = __builtin_amdgcn_readfirstlane(s.n), tid = threadIdx.x;
     gc = __builtin_amdgcn_readfirstlane(gc);
-    // pass 1: r = b - op(A) x (and the row sums of |op(A)|), 4 columns of the
-    // operators in flight per round
+    // pass 1: r = b - op(A) x (and the row sums of |op(A)|), kResCols columns of
+    // the operators in flight per round
     double amax = 0.0;
     for (int i = tid; i < n; i += kThreads) {
         z64 acc[kMaxGroup];
@@ -1089,35 +1090,49 @@ __device__ void residual(const SysDesc& s, double w, double w2, const double2* b
                                   : ldc<3>(bl + i + (size_t)g * n))
                             : czero<double>();
         double asum = 0.0;
-        for (int j0 = 0; j0 < n; j0 += 4) {
-            z64 a[4];
+        for (int j0 = 0; j0 < n; j0 += kResCols) {
             if (s.real_ops) {
                 const double* __restrict__ Kp = s.K;
                 const double* __restrict__ Mp = s.M;
                 const double* __restrict__ Dp = s.D;
+                double kv[kResCols], mv[kResCols], dv[kResCols];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kResCols; ++u) {
                     const int j = min(j0 + u, n - 1);
-                    const size_t e = ADJ ? (size_t)j + (size_t)i * n : (size_t)i + (size_t)j * n;
-                    const double k = lds1<1>(Kp + e), mm = lds1<1>(Mp + e);
-                    const double d = s.has_D ? lds1<1>(Dp + e) : 0.0;
-                    a[u] = cmk<double>(k - w2 * mm, (ADJ ? -1.0 : 1.0) * (s.eta * k + w * d));
+                    const int e = ADJ ? j + i * n : i + j * n;
+                    kv[u] = lds1<1>(Kp + e);
+                    mv[u] = lds1<1>(Mp + e);
+                    dv[u] = s.has_D ? lds1<1>(Dp + e) : 0.0;
                 }
-            } else {
+#pragma unroll
+                for (int u = 0; u < kResCols; ++u) {
+                    if (j0 + u < n) {
+                        const z64 au = cmk<double>(kv[u] - w2 * mv[u], (ADJ ? -1.0 : 1.0) * (s.eta * kv[u] + w * dv[u]));
+                        if (need_anorm) asum += zabs(au);
+#pragma unroll
+                        for (int g = 0; g < kMaxGroup; ++g)
+                            if (g < gc) cfms(acc[g], au, ldc<3>(xs + j0 + u + (size_t)g * n));
+                    }
+                }
+                continue;
+            }
+            // complex operators: 4 columns per round
+            for (int j1 = j0; j1 < j0 + kResCols && j1 < n; j1 += 4) {
+                z64 a[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int j = min(j0 + u, n - 1);
+                    const int j = min(j1 + u, n - 1);
                     a[u] = ADJ ? cconj(assemble_at(s, (size_t)j + (size_t)i * n, w, w2))
                                : assemble_at(s, (size_t)i + (size_t)j * n, w, w2);
                 }
-            }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (j0 + u < n) {
-                    if (need_anorm) asum += zabs(a[u]);
+                for (int u = 0; u < 4; ++u) {
+                    if (j1 + u < n) {
+                        if (need_anorm) asum += zabs(a[u]);
 #pragma unroll
-                    for (int g = 0; g < kMaxGroup; ++g)
-                        if (g < gc) cfms(acc[g], a[u], ldc<3>(xs + j0 + u + (size_t)g * n));
+                        for (int g = 0; g < kMaxGroup; ++g)
+                            if (g < gc) cfms(acc[g], a[u], ldc<3>(xs + j1 + u + (size_t)g * n));
+                    }
                 }
             }
         }
