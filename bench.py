@@ -37,7 +37,7 @@ PEAK_HBM_GBS = 8000.0
 # the same counters read for a 256 MiB device-to-device copy (FETCH_SIZE reads half
 # the bytes copied: x2.0; WRITE_SIZE reads them exactly: x1.0).  The summary and the
 # commands are in profiles/sweep_kernel_n256_pmc.txt; re-measure when the kernel changes.
-TRAFFIC_KIB = {"refined": (7.507e6, 11.024e6), "plain": (16.53e6, 30.77e6)}
+TRAFFIC_KIB = {"refined": (9.015e6, 12.28e6), "plain": (24.03e6, 42.66e6)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
