@@ -142,25 +142,17 @@ def cpu_baseline(case, budget_s=20.0):
     cores = max(1, min(cores, 16))
     rom = case.rom
     lib = oracle.openblas_path()
-    # calibrate on 64 frequencies, then size the sample to ~budget_s of wall time
-    nf = 64
-    if lib is not None:
-        r = oracle.sweep_lapack(rom.K, rom.M, rom.D, rom.eta, case.freq[:nf], case.F, nthreads=cores)
+    # one full 4096-frequency sweep calibrates; whole sweeps then repeat for ~budget_s
+    if lib is not None and oracle.sweep_lapack(rom.K, rom.M, rom.D, rom.eta, case.freq[:8], case.F,
+                                               nthreads=cores) is not None:
         kind_lib = "OpenBLAS (scipy-bundled) zgetrf_/zgetrs_"
+        fn = lambda fr: oracle.sweep_lapack(rom.K, rom.M, rom.D, rom.eta, fr, case.F,
+                                            nthreads=cores)[2]
     else:
-        r = None
-    if r is None:
-        _, _, _, t = oracle.sweep(rom.K, rom.M, rom.D, rom.eta, case.freq[:nf], case.F,
-                                  nthreads=cores, berr=False)
         kind_lib = "oracle C zgetf2/zgetrs"
         fn = lambda fr: oracle.sweep(rom.K, rom.M, rom.D, rom.eta, fr, case.F, nthreads=cores,
                                      berr=False)[3]
-    else:
-        t = r[2]
-        fn = lambda fr: oracle.sweep_lapack(rom.K, rom.M, rom.D, rom.eta, fr, case.F,
-                                            nthreads=cores)[2]
-    # full sweeps of all 4096 frequencies, repeated until ~budget_s of wall time
-    per_sweep = max(t / nf, 1e-9) * NFREQ
+    per_sweep = max(fn(case.freq), 1e-6)
     reps = int(max(1, min(200, budget_s / per_sweep)))
     t = 0.0
     for _ in range(reps):
@@ -355,6 +347,13 @@ def main():
     }
     if "plain" in res and "refined" in res:
         out["speedup_vs_plain_fp64"] = res["refined"]["pts_per_s"] / res["plain"]["pts_per_s"]
+    # host buffers in and out (stc_sweep: PCIe copies of F and X included); reported, never value
+    if "refined" in res:
+        h.sweep(0, case.freq, case.F, mode="refined", tol=TOL_REFINED, max_iter=10, berr=True)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            h.sweep(0, case.freq, case.F, mode="refined", tol=TOL_REFINED, max_iter=10, berr=True)
+        out["refined_host_buffers_pts_per_s"] = 3 * NFREQ / (time.perf_counter() - t0)
     if not args.no_vendor:
         try:
             v, verr = vendor_fp64(case, max(1, args.steps // 5), 1)
