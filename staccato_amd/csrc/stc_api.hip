@@ -98,19 +98,32 @@ int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
     return STC_OK;
 }
 
-constexpr int kBudgetSmall = 80 * 1024;      // two workgroups per CU
+constexpr int kBudgetThird = 53 * 1024;       // three workgroups per CU
+constexpr int kBudgetSmall = 80 * 1024;       // two workgroups per CU
 constexpr int kBudgetLarge = 160 * 1024 - 256;  // one workgroup per CU
 
-// Pick the LDS budget of a launch and fill each system's layout.
+// Pick the LDS budget of a launch and fill each system's layout: the largest
+// number of workgroups per CU the kernel is built for at which every blocked
+// system still gets a panel of at least 12 columns (a thin panel starves the
+// trailing update).
 int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int jb_cap, int* lds_out) {
-    int budget = kBudgetSmall;
-    if (const char* e = getenv("STC_LDS_BUDGET")) budget = std::max(16384, std::min(atoi(e), kBudgetLarge));
+    int budget = kBudgetLarge;
     if (const char* e = getenv("STC_JB_MAX")) jb_cap = std::max(4, std::min(atoi(e), jb_cap));
-    if (!getenv("STC_LDS_BUDGET")) {
-        // a thin panel starves the trailing update: take the whole CU instead
-        for (const SysDesc& s : sys) {
-            Layout L = plan_layout(s.n, s.nrhs, ts, kBudgetSmall, jb_cap, adj);
-            if (L.lds > kBudgetSmall || (!L.small && L.jb < 12)) budget = kBudgetLarge;
+    if (const char* e = getenv("STC_LDS_BUDGET")) {
+        budget = std::max(16384, std::min(atoi(e), kBudgetLarge));
+    } else {
+        const int cands[] = {kBudgetThird, kBudgetSmall};
+        for (int b : cands) {
+            if (b == kBudgetThird && wg_per_cu(ts) < 3) continue;
+            bool ok = true;
+            for (const SysDesc& s : sys) {
+                Layout L = plan_layout(s.n, s.nrhs, ts, b, jb_cap, adj);
+                if (L.lds > b || (!L.small && L.jb < 12)) ok = false;
+            }
+            if (ok) {
+                budget = b;
+                break;
+            }
         }
     }
     int lds = 0;
@@ -174,7 +187,7 @@ int configure(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int refin
     int r = (maxn + 63) / 64, maxr = 1;
     while (maxr < r) maxr <<= 1;
     cfg->maxr = maxr;
-    // the register-resident panel holds 8 / 4 / 2 columns per wavefront
+    // the register-resident panel holds 4 (or 2, 1) columns per wavefront
     const int jb_cap = kWaves * panel_cols_per_wave(refine ? 8 : 16, maxr);
     int rc = plan_systems(sys, ts, adj, jb_cap, &cfg->lds);
     if (rc) return rc;

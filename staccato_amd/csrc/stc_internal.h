@@ -93,11 +93,17 @@ constexpr int kUwLdCols = 17;     // padded row length of a wave's U12 chunk in 
 constexpr int kSolveBlk = 32;     // diagonal block of the blocked solves
 
 // Panel columns each wavefront keeps in registers (x MAXR rows per lane):
-// about 32 complex values per lane.
+// 16 complex values per lane for complex64 (the kernel runs three workgroups
+// per CU, at most 168 registers), 16 for complex128 up to n = 512 (two per CU).
 __host__ __device__ constexpr int panel_cols_per_wave(int elem_bytes, int maxr) {
-    return elem_bytes == 8 ? (maxr <= 4 ? 8 : (maxr <= 8 ? 4 : 2))
+    return elem_bytes == 8 ? (maxr <= 8 ? 4 : 2)
                            : (maxr <= 4 ? 4 : (maxr <= 8 ? 2 : 1));
 }
+
+// Workgroups (of four wavefronts) per CU the kernel is compiled for: three for
+// the complex64 factorisation, two for complex128 (its panel and products need
+// up to 256 registers).
+__host__ __device__ constexpr int wg_per_cu(int elem_bytes) { return elem_bytes == 8 ? 3 : 2; }
 
 // Elements of a blocked-path slot: the n x n factors plus the inverted
 // diagonal blocks of L and U (ceil(n/jb) blocks of jb x jb each).

@@ -613,14 +613,10 @@ __device__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int
             }
         }
     };
-    // complex64 keeps two C tiles and the next chunk's A12 in flight; the
-    // complex128 registers allow one tile ahead
-    constexpr bool kDeep = sizeof(T) == 4;
     cx<T> av[KS];
-    if (kDeep && wave < nch) load_a12(c0 + wave * kUw, min(kUw, n - c0 - wave * kUw), av);
     for (int ch = wave; ch < nch; ch += kWaves) {
         const int cb = c0 + ch * kUw, cw = min(kUw, n - cb);
-        if (!kDeep) load_a12(cb, cw, av);
+        load_a12(cb, cw, av);
         // ---- U12 = Linv A12 ----
         {
             acc_t ure[2], uim[2];
@@ -660,19 +656,16 @@ __device__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int
             }
         }
         __builtin_amdgcn_wave_barrier();
-        // ---- C -= L21 U12, tiles of 32 rows, loaded ahead of their products ----
-        acc_t cre[2], cim[2], bre[2], bim[2];
-        int prow[2], bprow[2];
+        // ---- C -= L21 U12, tiles of 32 rows, the next tile loaded before the
+        // current tile's products ----
+        acc_t cre[2], cim[2];
+        int prow[2];
         load_c(0, cb, cw, cre, cim, prow);
-        if (kDeep && 32 < m2) load_c(32, cb, cw, bre, bim, bprow);
-        // the next chunk's A12 (its rows are final: the U12 of this chunk only
-        // touches this chunk's columns)
-        if (kDeep && ch + kWaves < nch) load_a12(cb + kWaves * kUw, min(kUw, n - cb - kWaves * kUw), av);
         for (int r0 = 0; r0 < m2; r0 += 32) {
             acc_t nre[2], nim[2];
             int nprow[2];
-            const bool more2 = kDeep ? r0 + 64 < m2 : r0 + 32 < m2;
-            if (more2) load_c(kDeep ? r0 + 64 : r0 + 32, cb, cw, nre, nim, nprow);
+            const bool more = r0 + 32 < m2;
+            if (more) load_c(r0 + 32, cb, cw, nre, nim, nprow);
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
                 if (4 * s < jb) {
@@ -701,18 +694,9 @@ __device__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int
                     }
                 }
             }
+            if (more) {
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                if (kDeep) {
-                    cre[a] = bre[a];
-                    cim[a] = bim[a];
-                    prow[a] = bprow[a];
-                    if (more2) {
-                        bre[a] = nre[a];
-                        bim[a] = nim[a];
-                        bprow[a] = nprow[a];
-                    }
-                } else if (more2) {
+                for (int a = 0; a < 2; ++a) {
                     cre[a] = nre[a];
                     cim[a] = nim[a];
                     prow[a] = nprow[a];
@@ -1497,10 +1481,12 @@ __device__ __forceinline__ SysDesc load_uniform(const SysDesc* p) {
 }
 
 // ---- the persistent kernel -------------------------------------------------------------
-// Two workgroups (8 wavefronts) per CU: the register file is split two ways,
-// so each wavefront gets at most 256 registers.
+// wg_per_cu(sizeof(cx<T>)) workgroups of 4 wavefronts per CU: the register
+// file is split that many ways (at most 168 registers per wavefront for three,
+// 256 for two).
 template <typename T, int MAXR>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ void __launch_bounds__(kThreads)
+    __attribute__((amdgpu_waves_per_eu(wg_per_cu(sizeof(cx<T>)), wg_per_cu(sizeof(cx<T>)))))
 sweep_kernel(KParams kp) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ int s_item;
