@@ -307,7 +307,12 @@ struct Mfma<double> {
 };
 
 constexpr int kSolveBlock = 32; // diagonal block of the blocked triangular solves
-constexpr int kResCols = 16;    // operator columns per load round of the residual
+// Operator columns per load round of the residual: 8 in the complex64 kernel
+// (168 registers per lane), 16 in the complex128 one (256).
+template <typename T>
+struct ResCols {
+    static constexpr int K = sizeof(T) == 4 ? 8 : 16;
+};
 constexpr int kUw = kUwCols;    // columns of one wave's U12 chunk
 constexpr int kUwLd = kUwLdCols;
 
@@ -787,11 +792,11 @@ __device__ __forceinline__ const cx<T>* stage_diag(const TriCtx<T>& t, int k0, i
     return t.LU + k0 + (size_t)k0 * t.ld;
 }
 
-// Loads per round of the triangular-solve products: a whole diagonal block
-// (kb <= kSolveBlock) of complex64, half of one of complex128.
+// Loads per round of the triangular-solve products (16 complex values: what
+// the register budget of either kernel leaves room for).
 template <typename T>
 struct SolveRound {
-    static constexpr int K = sizeof(T) == 4 ? 32 : 16;
+    static constexpr int K = 16;
 };
 
 template <int LUAS, typename T, bool CONJ, bool TRANS>
@@ -1057,13 +1062,13 @@ __device__ void block_sum(double (&vals)[NS], double* red) {
 // need_anorm), then per column g: out[1+4g] = ||r||_inf, out[2+4g] = ||x||_inf,
 // out[3+4g] = ||b||_inf, out[4+4g] = max |re r| + |im r|.
 // b is global (ldb) when bg != nullptr, else LDS (bl, ld n).
-template <bool ADJ>
+template <bool ADJ, int RC>
 __device__ void residual(const SysDesc& s, double w, double w2, const double2* bg, int ldb,
                          const z64* bl, const z64* xs, z64* rs, int gc, bool need_anorm,
                          double* red, double (&out)[kRedSlots]) {
     const int n = __builtin_amdgcn_readfirstlane(s.n), tid = threadIdx.x;
     gc = __builtin_amdgcn_readfirstlane(gc);
-    // pass 1: r = b - op(A) x (and the row sums of |op(A)|), kResCols columns of
+    // pass 1: r = b - op(A) x (and the row sums of |op(A)|), RC columns of
     // the operators in flight per round
     double amax = 0.0;
     for (int i = tid; i < n; i += kThreads) {
@@ -1074,14 +1079,14 @@ __device__ void residual(const SysDesc& s, double w, double w2, const double2* b
                                   : ldc<3>(bl + i + (size_t)g * n))
                             : czero<double>();
         double asum = 0.0;
-        for (int j0 = 0; j0 < n; j0 += kResCols) {
+        for (int j0 = 0; j0 < n; j0 += RC) {
             if (s.real_ops) {
                 const double* __restrict__ Kp = s.K;
                 const double* __restrict__ Mp = s.M;
                 const double* __restrict__ Dp = s.D;
-                double kv[kResCols], mv[kResCols], dv[kResCols];
+                double kv[RC], mv[RC], dv[RC];
 #pragma unroll
-                for (int u = 0; u < kResCols; ++u) {
+                for (int u = 0; u < RC; ++u) {
                     const int j = min(j0 + u, n - 1);
                     const int e = ADJ ? j + i * n : i + j * n;
                     kv[u] = lds1<1>(Kp + e);
@@ -1089,7 +1094,7 @@ __device__ void residual(const SysDesc& s, double w, double w2, const double2* b
                     dv[u] = s.has_D ? lds1<1>(Dp + e) : 0.0;
                 }
 #pragma unroll
-                for (int u = 0; u < kResCols; ++u) {
+                for (int u = 0; u < RC; ++u) {
                     if (j0 + u < n) {
                         const z64 au = cmk<double>(kv[u] - w2 * mv[u], (ADJ ? -1.0 : 1.0) * (s.eta * kv[u] + w * dv[u]));
                         if (need_anorm) asum += zabs(au);
@@ -1101,7 +1106,7 @@ __device__ void residual(const SysDesc& s, double w, double w2, const double2* b
                 continue;
             }
             // complex operators: 4 columns per round
-            for (int j1 = j0; j1 < j0 + kResCols && j1 < n; j1 += 4) {
+            for (int j1 = j0; j1 < j0 + RC && j1 < n; j1 += 4) {
                 z64 a[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -1190,7 +1195,7 @@ __device__ int solve_group(const SysDesc& s, double w, double w2, const TriCtx<T
     if (!want_berr) return 0;
     for (;;) {
         double out[kRedSlots];
-        residual<ADJ>(s, w, w2, bg, ldb, bl, xs, rs, gc, *anorm < 0.0, red, out);
+        residual<ADJ, ResCols<T>::K>(s, w, w2, bg, ldb, bl, xs, rs, gc, *anorm < 0.0, red, out);
         if (*anorm < 0.0) *anorm = out[0];
         bool conv = true;
 #pragma unroll
