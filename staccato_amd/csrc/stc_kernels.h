@@ -577,7 +577,7 @@ __device__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<T>* Linv_lds, cx
 //   U12 = Linv A12 on MFMA (A12 = logical rows [j0, j0+jb) of the chunk, all
 //   of its loads in flight together), written to S and to the wave's LDS
 //   buffer Uw (jb x kUw, row-major, ld kUwLd);
-//   C -= L21 U12 for logical rows [j0+jb, n) in tiles of 32 rows, the next
+//   C -= L21 U12 for logical rows [j0+jb, n) in tiles of 64 rows, the next
 //   tile's C loaded before the current tile's products.
 // Both products are computed transposed so that the MFMA operands come
 // straight from the column-major slot and LDS panel: D = A12^T Linv^T = U12^T
@@ -589,6 +589,7 @@ __device__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int
     using M = Mfma<T>;
     using acc_t = typename M::acc;
     constexpr int KS = 8;  // k-steps of 4 for jb <= 32
+    constexpr int kTA = 4; // 16-row MFMA tiles per C tile (64 rows)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lg = lane >> 4;
     const int c0 = j0 + jb, m2 = n - c0;
@@ -602,9 +603,9 @@ __device__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int
                                         : czero<T>();
         }
     };
-    auto load_c = [&](int r0, int cb, int cw, acc_t (&re)[2], acc_t (&im)[2], int (&pr)[2]) {
+    auto load_c = [&](int r0, int cb, int cw, acc_t (&re)[kTA], acc_t (&im)[kTA], int (&pr)[kTA]) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
+        for (int a = 0; a < kTA; ++a) {
             const int row = r0 + 16 * a + li;
             const bool ok = row < m2;
             pr[a] = ok ? lds1<3>(rmap + c0 + row) : 0;
@@ -661,16 +662,16 @@ __device__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int
             }
         }
         __builtin_amdgcn_wave_barrier();
-        // ---- C -= L21 U12, tiles of 32 rows, the next tile loaded before the
-        // current tile's products ----
-        acc_t cre[2], cim[2];
-        int prow[2];
+        // ---- C -= L21 U12, tiles of 16 kTA rows, the next tile loaded before
+        // the current tile's products ----
+        acc_t cre[kTA], cim[kTA];
+        int prow[kTA];
         load_c(0, cb, cw, cre, cim, prow);
-        for (int r0 = 0; r0 < m2; r0 += 32) {
-            acc_t nre[2], nim[2];
-            int nprow[2];
-            const bool more = r0 + 32 < m2;
-            if (more) load_c(r0 + 32, cb, cw, nre, nim, nprow);
+        for (int r0 = 0; r0 < m2; r0 += 16 * kTA) {
+            acc_t nre[kTA], nim[kTA];
+            int nprow[kTA];
+            const bool more = r0 + 16 * kTA < m2;
+            if (more) load_c(r0 + 16 * kTA, cb, cw, nre, nim, nprow);
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
                 if (4 * s < jb) {
@@ -678,18 +679,20 @@ __device__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int
                     const bool kok = k < jb;
                     const cx<T> uv = kok ? ldc<3>(Uw + k * kUwLd + li) : czero<T>();
 #pragma unroll
-                    for (int a = 0; a < 2; ++a) {
-                        const int row = min(r0 + 16 * a + li, m2 - 1);
-                        const cx<T> lv = kok ? ldc<3>(P + ((jb + row) + k * ldp)) : czero<T>();
-                        cre[a] = M::run(-uv.re, lv.re, cre[a]);
-                        cre[a] = M::run(uv.im, lv.im, cre[a]);
-                        cim[a] = M::run(-uv.re, lv.im, cim[a]);
-                        cim[a] = M::run(-uv.im, lv.re, cim[a]);
+                    for (int a = 0; a < kTA; ++a) {
+                        if (r0 + 16 * a < m2) {
+                            const int row = min(r0 + 16 * a + li, m2 - 1);
+                            const cx<T> lv = kok ? ldc<3>(P + ((jb + row) + k * ldp)) : czero<T>();
+                            cre[a] = M::run(-uv.re, lv.re, cre[a]);
+                            cre[a] = M::run(uv.im, lv.im, cre[a]);
+                            cim[a] = M::run(-uv.re, lv.im, cim[a]);
+                            cim[a] = M::run(-uv.im, lv.re, cim[a]);
+                        }
                     }
                 }
             }
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
+            for (int a = 0; a < kTA; ++a) {
                 const int row = r0 + 16 * a + li;
                 if (row < m2) {
 #pragma unroll
@@ -701,7 +704,7 @@ __device__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int
             }
             if (more) {
 #pragma unroll
-                for (int a = 0; a < 2; ++a) {
+                for (int a = 0; a < kTA; ++a) {
                     cre[a] = nre[a];
                     cim[a] = nim[a];
                     prow[a] = nprow[a];
