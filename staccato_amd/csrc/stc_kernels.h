@@ -202,7 +202,7 @@ __device__ __forceinline__ int wave_argmax(const cx<T>* col, int start, int m) {
 // the composite row permutation) gets the same interchange; sc[0] = pivot
 // slot, sc[1] = first zero pivot (global, 1-based) or 0.
 template <typename T, int MAXR>
-__device__ void panel_factor(cx<T>* P, int ldp, int m, int jb, int j0, int* ipiv,
+__device__ __forceinline__ void panel_factor(cx<T>* P, int ldp, int m, int jb, int j0, int* ipiv,
                              int* rmap, int* sc) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (wave == 0) {
@@ -414,7 +414,7 @@ __device__ __forceinline__ void rotate_cols(cx<T> (&x)[CPW][MAXR]) {
 }
 
 template <typename T, int MAXR, int CPW>
-__device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, int jb, int* ipiv,
+__device__ __forceinline__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, int jb, int* ipiv,
                                  int* rmap, int* sc, cx<T>* lcolbuf, cx<T>* P) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = n - j0;
@@ -536,7 +536,7 @@ __device__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, int n, int j0, i
 // to LDS (ld jb) for the TRSM and, with Uinv, to the slot's block store
 // (block column-major, ld jb) for the triangular solves.
 template <typename T, int CPW>
-__device__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<T>* Linv_lds, cx<T>* __restrict__ Lg,
+__device__ __forceinline__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<T>* Linv_lds, cx<T>* __restrict__ Lg,
                             cx<T>* __restrict__ Ug) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     cx<T> xl[CPW], xu[CPW];
@@ -584,7 +584,7 @@ __device__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<T>* Linv_lds, cx
 // and D = U12^T L21^T = (L21 U12)^T.  Complex products take 4 real MFMAs.
 // jb <= 32.
 template <typename T>
-__device__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb, int n,
+__device__ __forceinline__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb, int n,
                              const cx<T>* P, int ldp, const cx<T>* Linv, cx<T>* Uw_all) {
     using M = Mfma<T>;
     using acc_t = typename M::acc;
@@ -722,7 +722,7 @@ __device__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int
 // holds the inverted diagonal blocks (Linv, Uinv; jb x jb each per panel),
 // used by the triangular solves.
 template <typename T, int MAXR>
-__device__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* __restrict__ S,
+__device__ __forceinline__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* __restrict__ S,
                                const Layout& lay, const Lds& L) {
     constexpr int CPW = panel_cols_per_wave((int)sizeof(cx<T>), MAXR);
     const int n = s.n, tid = threadIdx.x;
@@ -884,7 +884,7 @@ __device__ __forceinline__ void block_inv_apply(const cx<T>* __restrict__ Bi, in
 
 // L unit lower, forward, with the inverted diagonal blocks.
 template <typename T>
-__device__ void lower_unit_solve_inv(const TriCtx<T>& t, cx<T>* w, int gc) {
+__device__ __forceinline__ void lower_unit_solve_inv(const TriCtx<T>& t, cx<T>* w, int gc) {
     const int tid = threadIdx.x, n = t.n;
     for (int k0 = 0, b = 0; k0 < n; k0 += t.bs, ++b) {
         const int kb = min(t.bs, n - k0);
@@ -897,7 +897,7 @@ __device__ void lower_unit_solve_inv(const TriCtx<T>& t, cx<T>* w, int gc) {
 
 // U upper, backward, with the inverted diagonal blocks.
 template <typename T>
-__device__ void upper_solve_inv(const TriCtx<T>& t, cx<T>* w, int gc) {
+__device__ __forceinline__ void upper_solve_inv(const TriCtx<T>& t, cx<T>* w, int gc) {
     const int tid = threadIdx.x, n = t.n;
     const int nb = (n + t.bs - 1) / t.bs;
     for (int b = nb - 1; b >= 0; --b) {
@@ -911,7 +911,7 @@ __device__ void upper_solve_inv(const TriCtx<T>& t, cx<T>* w, int gc) {
 
 // L unit lower, forward.
 template <typename T>
-__device__ void lower_unit_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
+__device__ __forceinline__ void lower_unit_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, n = t.n;
     for (int k0 = 0; k0 < n; k0 += kSolveBlock) {
         const int kb = min(kSolveBlock, n - k0);
@@ -935,7 +935,7 @@ __device__ void lower_unit_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
 
 // U upper, backward.
 template <typename T>
-__device__ void upper_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
+__device__ __forceinline__ void upper_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, n = t.n;
     const int nb = (n + kSolveBlock - 1) / kSolveBlock;
     for (int b = nb - 1; b >= 0; --b) {
@@ -961,7 +961,7 @@ __device__ void upper_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
 
 // U^H y = b, forward (adjoint).
 template <typename T>
-__device__ void upper_h_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
+__device__ __forceinline__ void upper_h_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, n = t.n;
     for (int k0 = 0; k0 < n; k0 += kSolveBlock) {
         const int kb = min(kSolveBlock, n - k0);
@@ -986,7 +986,7 @@ __device__ void upper_h_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
 
 // L^H v = y, backward, unit diagonal (adjoint).
 template <typename T>
-__device__ void lower_h_unit_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
+__device__ __forceinline__ void lower_h_unit_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, n = t.n;
     const int nb = (n + kSolveBlock - 1) / kSolveBlock;
     for (int b = nb - 1; b >= 0; --b) {
@@ -1066,7 +1066,7 @@ __device__ void block_sum(double (&vals)[NS], double* red) {
 // out[3+4g] = ||b||_inf, out[4+4g] = max |re r| + |im r|.
 // b is global (ldb) when bg != nullptr, else LDS (bl, ld n).
 template <bool ADJ, int RC>
-__device__ void residual(const SysDesc& s, double w, double w2, const double2* bg, int ldb,
+__device__ __forceinline__ void residual(const SysDesc& s, double w, double w2, const double2* bg, int ldb,
                          const z64* bl, const z64* xs, z64* rs, int gc, bool need_anorm,
                          double* red, double (&out)[kRedSlots]) {
     const int n = __builtin_amdgcn_readfirstlane(s.n), tid = threadIdx.x;
@@ -1168,7 +1168,7 @@ __device__ __forceinline__ double backward_error(double rmax, double anorm, doub
 // On exit xs holds the solution (FP64).  Returns refinement steps; conv[] = 1
 // for columns meeting tol; berr_out[] their backward errors (when computed).
 template <typename T, bool ADJ>
-__device__ int solve_group(const SysDesc& s, double w, double w2, const TriCtx<T>& tri,
+__device__ __forceinline__ int solve_group(const SysDesc& s, double w, double w2, const TriCtx<T>& tri,
                            const int* perm, cx<T>* wv, z64* xs, z64* rs, int gc,
                            const double2* bg, int ldb, const z64* bl, bool refine, bool want_berr,
                            double tol, int max_iter, double* anorm,
@@ -1255,7 +1255,7 @@ __device__ int solve_group(const SysDesc& s, double w, double w2, const TriCtx<T
 // = {J_w, dJ/deta_w, dJ/ds_0,w, ...} and info[fidx] (0, or -1 when a refined
 // solve missed tol).
 template <typename T>
-__device__ void adjoint_point(const SysDesc& s, int fidx, double w, double w2, const TriCtx<T>& tri,
+__device__ __forceinline__ void adjoint_point(const SysDesc& s, int fidx, double w, double w2, const TriCtx<T>& tri,
                               const Lds& L, cx<T>* wv, z64* xs, z64* rs, z64* xk, z64* bv,
                               bool refine, double tol, int max_iter) {
     const int n = s.n, tid = threadIdx.x, p = s.p;
@@ -1342,7 +1342,7 @@ __device__ void adjoint_point(const SysDesc& s, int fidx, double w, double w2, c
 
 // ---- one point ---------------------------------------------------------------------------
 template <typename T, int MAXR>
-__device__ void process_point(const SysDesc& s, int fidx, const KParams& kp, char* smem) {
+__device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const KParams& kp, char* smem) {
     const int n = s.n, tid = threadIdx.x, nrhs = s.nrhs;
     const double w = kTwoPi * s.freq[fidx], w2 = w * w;
     Layout lay;
