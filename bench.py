@@ -33,11 +33,11 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the sweep kernel on this workload: rocprofv3 --pmc passes
-# (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean of 4 launches) scaled by what
+# (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean over the launches) scaled by what
 # the same counters read for a 256 MiB device-to-device copy (FETCH_SIZE reads half
 # the bytes copied: x2.0; WRITE_SIZE reads them exactly: x1.0).  The summary and the
 # commands are in profiles/sweep_kernel_n256_pmc.txt; re-measure when the kernel changes.
-TRAFFIC_KIB = {"refined": (9.015e6, 12.28e6), "plain": (24.03e6, 42.66e6)}
+TRAFFIC_KIB = {"refined": (10.62e6, 15.82e6), "plain": (21.61e6, 38.23e6)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
