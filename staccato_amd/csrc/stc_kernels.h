@@ -115,6 +115,18 @@ __device__ __forceinline__ cx<T> crecip(cx<T> z) {
     T d = z.im + z.re * r;
     return cmk<T>(r / d, T(-1) / d);
 }
+// complex64 (the factor of the refined mode, whose rounding the FP64
+// refinement absorbs): Smith's scaling with one hardware reciprocal
+// (v_rcp_f32, 1 ulp) instead of three correctly rounded divisions.
+template <>
+__device__ __forceinline__ cx<float> crecip(cx<float> z) {
+    const bool re_big = fabsf(z.re) >= fabsf(z.im);
+    const float a = re_big ? z.re : z.im, b = re_big ? z.im : z.re;
+    const float ia = __builtin_amdgcn_rcpf(a);
+    const float r = b * ia;
+    const float id = __builtin_amdgcn_rcpf(a + b * r);
+    return re_big ? cmk<float>(id, -r * id) : cmk<float>(r * id, -id);
+}
 template <typename T>
 __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) { return cmul(a, crecip(b)); }
 template <typename T>
@@ -461,7 +473,7 @@ __device__ __forceinline__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, 
 #pragma unroll
             for (int t = 1; t < MAXR; ++t)
                 if (ps == t) pv = x[0][t];
-            pv = cshfl(pv, pln);
+            pv = cbcast(pv, pln);
             const bool zero = pv.re == T(0) && pv.im == T(0);
             const cx<T> rp = zero ? czero<T>() : crecip(pv);
 #pragma unroll
