@@ -542,35 +542,23 @@ __device__ __forceinline__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, 
         }
 }
 
-// ---- inverses of the diagonal block L11 (unit lower) and U11 (upper) ----------------------
-// From P[0:jb, 0:jb] (LDS, ld ldp).  Wave w computes columns w, w+4, ... of
-// both, all of its columns interleaved (lanes = rows, jb <= 64).  Linv goes
-// to LDS (ld jb) for the TRSM and, with Uinv, to the slot's block store
-// (block column-major, ld jb) for the triangular solves.
+// ---- inverse of the diagonal block L11 (unit lower) ----------------------------------------
+// From P[0:jb, 0:jb] (LDS, ld ldp).  Wave w computes columns w, w+4, ... (lanes
+// = rows, jb <= 64).  Linv goes to LDS (ld jb) for the TRSM and to the slot's
+// block store (block column-major, ld jb) for the triangular solves.
 template <typename T, int CPW>
-__device__ __forceinline__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<T>* Linv_lds, cx<T>* __restrict__ Lg,
-                            cx<T>* __restrict__ Ug) {
+__device__ __forceinline__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<T>* Linv_lds, cx<T>* __restrict__ Lg) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    cx<T> xl[CPW], xu[CPW];
+    cx<T> xl[CPW];
 #pragma unroll
     for (int q = 0; q < CPW; ++q) {
         const int j = wave + kWaves * q;
         xl[q] = lane == j ? cmk<T>(T(1), T(0)) : czero<T>();
-        xu[q] = xl[q];
     }
     for (int k = 0; k < jb - 1; ++k) {
         const cx<T> lk = (lane > k && lane < jb) ? P[lane + (size_t)k * ldp] : czero<T>();
 #pragma unroll
         for (int q = 0; q < CPW; ++q) cfms(xl[q], lk, cbcast(xl[q], k));
-    }
-    for (int k = jb - 1; k >= 0; --k) {
-        const cx<T> rk = crecip(P[k + (size_t)k * ldp]);
-        const cx<T> uk = (lane < k) ? P[lane + (size_t)k * ldp] : czero<T>();
-#pragma unroll
-        for (int q = 0; q < CPW; ++q) {
-            if (lane == k) xu[q] = cmul(xu[q], rk);
-            cfms(xu[q], uk, cbcast(xu[q], k));
-        }
     }
 #pragma unroll
     for (int q = 0; q < CPW; ++q) {
@@ -578,8 +566,46 @@ __device__ __forceinline__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<
         if (j < jb && lane < jb) {
             Linv_lds[lane + j * jb] = xl[q];
             Lg[lane + (size_t)j * jb] = xl[q];
-            Ug[lane + (size_t)j * jb] = xu[q];
         }
+    }
+}
+
+// ---- inverses of the diagonal blocks of U, after the factorisation -----------------------
+// Only the refined mode's block solves use them.  Wave w inverts blocks w,
+// w+4, ... (lanes = rows, bs <= 16) from the final factors in the slot, so no
+// block barrier is needed; the result goes to the block store (ld = block width).
+template <typename T>
+__device__ __forceinline__ void uinv_blocks(const cx<T>* __restrict__ S, int n, const int* rmap, int bs,
+                                            cx<T>* __restrict__ Ug) {
+    constexpr int B = 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nb = (n + bs - 1) / bs;
+    for (int b = wave; b < nb; b += kWaves) {
+        const int k0 = b * bs, kb = min(bs, n - k0);
+        const int pr = lds1<3>(rmap + k0 + min(lane, kb - 1));
+        cx<T> u[B], x[B];
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            u[c] = ldc<1>(S + (pr + (k0 + min(c, kb - 1)) * n));
+            x[c] = lane == c ? cmk<T>(T(1), T(0)) : czero<T>();
+        }
+#pragma unroll
+        for (int k = B - 1; k >= 0; --k) {
+            if (k < kb) {
+                const cx<T> rk = crecip(cbcast(u[k], k));
+                const cx<T> uk = lane < k ? u[k] : czero<T>();
+#pragma unroll
+                for (int c = 0; c < B; ++c) {
+                    if (c < kb) {
+                        if (lane == k) x[c] = cmul(x[c], rk);
+                        cfms(x[c], uk, cbcast(x[c], k));
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < B; ++c)
+            if (c < kb && lane < kb) stc<1>(Ug + ((size_t)b * bs * bs + lane + c * kb), x[c]);
     }
 }
 
@@ -759,10 +785,14 @@ __device__ __forceinline__ void factor_blocked(const SysDesc& s, double w, doubl
         const int jb = min(jbmax, n - j0), m = n - j0;
         panel_factor_reg<T, MAXR, CPW>(S, n, n, j0, jb, L.ipiv, L.rmap, L.sc, lcol, P);
         __syncthreads();
-        invert_diag<T, CPW>(P, m, jb, Linv, Lg + (size_t)blk * jbmax * jbmax,
-                            Ug + (size_t)blk * jbmax * jbmax);
+        invert_diag<T, CPW>(P, m, jb, Linv, Lg + (size_t)blk * jbmax * jbmax);
         __syncthreads();
         block_update(S, n, L.rmap, j0, jb, n, P, m, Linv, Uw);
+        __syncthreads();
+    }
+    // the complex64 (refined) solves apply inverted U blocks; complex128 solves substitute
+    if (sizeof(T) == 4) {
+        uinv_blocks(S, n, L.rmap, jbmax, Ug);
         __syncthreads();
     }
 }
