@@ -621,9 +621,13 @@ __device__ __forceinline__ void uinv_blocks(const cx<T>* __restrict__ S, int n, 
 // straight from the column-major slot and LDS panel: D = A12^T Linv^T = U12^T
 // and D = U12^T L21^T = (L21 U12)^T.  Complex products take 4 real MFMAs.
 // jb <= 32.
-template <typename T>
-__device__ __forceinline__ void block_update(cx<T>* __restrict__ S, int ld, const int* rmap, int j0, int jb, int n,
-                             const cx<T>* P, int ldp, const cx<T>* Linv, cx<T>* Uw_all) {
+// ASM: first block step, whose A12 and C have not been written to the slot
+// yet: they are assembled from the operators (physical row r, column c of A(w)
+// is element r + c n) and the results stored as usual.
+template <typename T, bool ASM>
+__device__ __forceinline__ void block_update(const SysDesc& sd, double w, double w2, cx<T>* __restrict__ S,
+                                             int ld, const int* rmap, int j0, int jb, int n,
+                                             const cx<T>* P, int ldp, const cx<T>* Linv, cx<T>* Uw_all) {
     using M = Mfma<T>;
     using acc_t = typename M::acc;
     constexpr int KS = 8;  // k-steps of 4 for jb <= 32
@@ -637,8 +641,13 @@ __device__ __forceinline__ void block_update(cx<T>* __restrict__ S, int ld, cons
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const int k = 4 * s + lg;
-            av[s] = (k < jb && li < cw) ? ldc<1>(S + (lds1<3>(rmap + j0 + k) + (cb + li) * ld))
-                                        : czero<T>();
+            if (ASM)
+                av[s] = (k < jb && li < cw)
+                            ? to_t<T>(assemble_at(sd, (size_t)lds1<3>(rmap + j0 + k) + (size_t)(cb + li) * ld, w, w2))
+                            : czero<T>();
+            else
+                av[s] = (k < jb && li < cw) ? ldc<1>(S + (lds1<3>(rmap + j0 + k) + (cb + li) * ld))
+                                            : czero<T>();
         }
     };
     auto load_c = [&](int r0, int cb, int cw, acc_t (&re)[kTA], acc_t (&im)[kTA], int (&pr)[kTA]) {
@@ -651,7 +660,9 @@ __device__ __forceinline__ void block_update(cx<T>* __restrict__ S, int ld, cons
             for (int r = 0; r < 4; ++r) {
                 const int col = M::row(lg, r);
                 cx<T> v = czero<T>();
-                if (ok && col < cw) v = ldc<1>(S + (pr[a] + (cb + col) * ld));
+                if (ok && col < cw)
+                    v = ASM ? to_t<T>(assemble_at(sd, (size_t)pr[a] + (size_t)(cb + col) * ld, w, w2))
+                            : ldc<1>(S + (pr[a] + (cb + col) * ld));
                 re[a][r] = v.re;
                 im[a][r] = v.im;
             }
@@ -709,7 +720,7 @@ __device__ __forceinline__ void block_update(cx<T>* __restrict__ S, int ld, cons
             acc_t nre[kTA], nim[kTA];
             int nprow[kTA];
             const bool more = r0 + 16 * kTA < m2;
-            if (more) load_c(r0 + 16 * kTA, cb, cw, nre, nim, nprow);
+            if (more && !ASM) load_c(r0 + 16 * kTA, cb, cw, nre, nim, nprow);
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
                 if (4 * s < jb) {
@@ -741,11 +752,15 @@ __device__ __forceinline__ void block_update(cx<T>* __restrict__ S, int ld, cons
                 }
             }
             if (more) {
+                if (ASM) {
+                    load_c(r0 + 16 * kTA, cb, cw, cre, cim, prow);
+                } else {
 #pragma unroll
-                for (int a = 0; a < kTA; ++a) {
-                    cre[a] = nre[a];
-                    cim[a] = nim[a];
-                    prow[a] = nprow[a];
+                    for (int a = 0; a < kTA; ++a) {
+                        cre[a] = nre[a];
+                        cim[a] = nim[a];
+                        prow[a] = nprow[a];
+                    }
                 }
             }
         }
@@ -765,8 +780,13 @@ __device__ __forceinline__ void factor_blocked(const SysDesc& s, double w, doubl
     constexpr int CPW = panel_cols_per_wave((int)sizeof(cx<T>), MAXR);
     const int n = s.n, tid = threadIdx.x;
     const size_t nn = (size_t)n * n;
+    // A(w) is assembled in FP64 and rounded to T where it is first needed: the
+    // first panel's columns here, every other column by the first block update
+    // (its A12 and C tiles come straight from the operators), so the matrix is
+    // never written to the slot and read back unmodified
+    const int nfirst = n * min(lay.jb, n);
 #pragma unroll 16
-    for (int e = tid; e < (int)nn; e += kThreads) stc<1>(S + e, to_t<T>(assemble_at(s, e, w, w2)));
+    for (int e = tid; e < nfirst; e += kThreads) stc<1>(S + e, to_t<T>(assemble_at(s, e, w, w2)));
     for (int r = tid; r < n; r += kThreads) L.rmap[r] = r;
     if (tid == 0) L.sc[1] = 0;
     __syncthreads();
@@ -787,7 +807,8 @@ __device__ __forceinline__ void factor_blocked(const SysDesc& s, double w, doubl
         __syncthreads();
         invert_diag<T, CPW>(P, m, jb, Linv, Lg + (size_t)blk * jbmax * jbmax);
         __syncthreads();
-        block_update(S, n, L.rmap, j0, jb, n, P, m, Linv, Uw);
+        if (j0 == 0) block_update<T, true>(s, w, w2, S, n, L.rmap, j0, jb, n, P, m, Linv, Uw);
+        else block_update<T, false>(s, w, w2, S, n, L.rmap, j0, jb, n, P, m, Linv, Uw);
         __syncthreads();
     }
     // the complex64 (refined) solves apply inverted U blocks; complex128 solves substitute
