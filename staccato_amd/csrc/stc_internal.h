@@ -105,11 +105,12 @@ __host__ __device__ constexpr int panel_cols_per_wave(int elem_bytes, int maxr) 
 // up to 256 registers).
 __host__ __device__ constexpr int wg_per_cu(int elem_bytes) { return elem_bytes == 8 ? 3 : 2; }
 
-// Elements of a blocked-path slot: the n x n factors plus the inverted
-// diagonal blocks of L and U (ceil(n/jb) blocks of jb x jb each).
+// Elements of a blocked-path slot: the n x n factors, up to kMaxGroup
+// right-hand-side columns eliminated along with them (columns n .. n+3, ld n),
+// and the inverted diagonal blocks of L and U (ceil(n/jb) blocks of jb x jb each).
 __host__ __device__ inline size_t slot_elems_for(int n, int jb) {
     const size_t nb = (size_t)((n + jb - 1) / jb);
-    return (size_t)n * n + 2 * nb * jb * jb;
+    return (size_t)n * (n + kMaxGroup) + 2 * nb * jb * jb;
 }
 
 // Fixed region at the start of LDS: ipiv and the row map (n ints each),

@@ -626,7 +626,7 @@ __device__ __forceinline__ void uinv_blocks(const cx<T>* __restrict__ S, int n, 
 // is element r + c n) and the results stored as usual.
 template <typename T, bool ASM>
 __device__ __forceinline__ void block_update(const SysDesc& sd, double w, double w2, cx<T>* __restrict__ S,
-                                             int ld, const int* rmap, int j0, int jb, int n,
+                                             int ld, const int* rmap, int j0, int jb, int n, int ncol,
                                              const cx<T>* P, int ldp, const cx<T>* Linv, cx<T>* Uw_all) {
     using M = Mfma<T>;
     using acc_t = typename M::acc;
@@ -634,14 +634,14 @@ __device__ __forceinline__ void block_update(const SysDesc& sd, double w, double
     constexpr int kTA = 4; // 16-row MFMA tiles per C tile (64 rows)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lg = lane >> 4;
-    const int c0 = j0 + jb, m2 = n - c0;
-    const int nch = (m2 + kUw - 1) / kUw;
+    const int c0 = j0 + jb, m2 = n - c0;            // rows of the trailing update
+    const int nch = (ncol - c0 + kUw - 1) / kUw;    // columns: matrix, then right-hand sides
     cx<T>* Uw = Uw_all + wave * jb * kUwLd;
     auto load_a12 = [&](int cb, int cw, cx<T> (&av)[KS]) {
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const int k = 4 * s + lg;
-            if (ASM)
+            if (ASM && cb + li < n)
                 av[s] = (k < jb && li < cw)
                             ? to_t<T>(assemble_at(sd, (size_t)lds1<3>(rmap + j0 + k) + (size_t)(cb + li) * ld, w, w2))
                             : czero<T>();
@@ -661,7 +661,8 @@ __device__ __forceinline__ void block_update(const SysDesc& sd, double w, double
                 const int col = M::row(lg, r);
                 cx<T> v = czero<T>();
                 if (ok && col < cw)
-                    v = ASM ? to_t<T>(assemble_at(sd, (size_t)pr[a] + (size_t)(cb + col) * ld, w, w2))
+                    v = (ASM && cb + col < n)
+                            ? to_t<T>(assemble_at(sd, (size_t)pr[a] + (size_t)(cb + col) * ld, w, w2))
                             : ldc<1>(S + (pr[a] + (cb + col) * ld));
                 re[a][r] = v.re;
                 im[a][r] = v.im;
@@ -670,7 +671,7 @@ __device__ __forceinline__ void block_update(const SysDesc& sd, double w, double
     };
     cx<T> av[KS];
     for (int ch = wave; ch < nch; ch += kWaves) {
-        const int cb = c0 + ch * kUw, cw = min(kUw, n - cb);
+        const int cb = c0 + ch * kUw, cw = min(kUw, ncol - cb);
         load_a12(cb, cw, av);
         // ---- U12 = Linv A12 ----
         {
@@ -776,7 +777,7 @@ __device__ __forceinline__ void block_update(const SysDesc& sd, double w, double
 // used by the triangular solves.
 template <typename T, int MAXR>
 __device__ __forceinline__ void factor_blocked(const SysDesc& s, double w, double w2, cx<T>* __restrict__ S,
-                               const Layout& lay, const Lds& L) {
+                                               const Layout& lay, const Lds& L, int nr) {
     constexpr int CPW = panel_cols_per_wave((int)sizeof(cx<T>), MAXR);
     const int n = s.n, tid = threadIdx.x;
     const size_t nn = (size_t)n * n;
@@ -787,6 +788,12 @@ __device__ __forceinline__ void factor_blocked(const SysDesc& s, double w, doubl
     const int nfirst = n * min(lay.jb, n);
 #pragma unroll 16
     for (int e = tid; e < nfirst; e += kThreads) stc<1>(S + e, to_t<T>(assemble_at(s, e, w, w2)));
+    // the first nr right-hand sides ride along as columns n .. n+nr-1: the
+    // block updates perform their forward elimination (L y = P b)
+    for (int e = tid; e < n * nr; e += kThreads) {
+        const int r = e % n, g = e / n;
+        stc<1>(S + (nn + (size_t)g * n + r), to_t<T>(ldc<1>(reinterpret_cast<const z64*>(s.F) + r + (size_t)g * s.ldf)));
+    }
     for (int r = tid; r < n; r += kThreads) L.rmap[r] = r;
     if (tid == 0) L.sc[1] = 0;
     __syncthreads();
@@ -798,7 +805,7 @@ __device__ __forceinline__ void factor_blocked(const SysDesc& s, double w, doubl
                                           align16((long long)kWaves * jbmax * kUwLd * sizeof(cx<T>)));
     cx<T>* lcol = P;  // the multiplier column lives in the panel region while the panel is in registers
     const size_t nblk = (size_t)((n + jbmax - 1) / jbmax);
-    cx<T>* Lg = S + nn;
+    cx<T>* Lg = S + (size_t)n * (n + kMaxGroup);
     cx<T>* Ug = Lg + nblk * jbmax * jbmax;
 
     for (int j0 = 0, blk = 0; j0 < n; j0 += jbmax, ++blk) {
@@ -807,8 +814,8 @@ __device__ __forceinline__ void factor_blocked(const SysDesc& s, double w, doubl
         __syncthreads();
         invert_diag<T, CPW>(P, m, jb, Linv, Lg + (size_t)blk * jbmax * jbmax);
         __syncthreads();
-        if (j0 == 0) block_update<T, true>(s, w, w2, S, n, L.rmap, j0, jb, n, P, m, Linv, Uw);
-        else block_update<T, false>(s, w, w2, S, n, L.rmap, j0, jb, n, P, m, Linv, Uw);
+        if (j0 == 0) block_update<T, true>(s, w, w2, S, n, L.rmap, j0, jb, n, n + nr, P, m, Linv, Uw);
+        else block_update<T, false>(s, w, w2, S, n, L.rmap, j0, jb, n, n + nr, P, m, Linv, Uw);
         __syncthreads();
     }
     // the complex64 (refined) solves apply inverted U blocks; complex128 solves substitute
@@ -1235,11 +1242,15 @@ __device__ __forceinline__ int solve_group(const SysDesc& s, double w, double w2
                            const int* perm, cx<T>* wv, z64* xs, z64* rs, int gc,
                            const double2* bg, int ldb, const z64* bl, bool refine, bool want_berr,
                            double tol, int max_iter, double* anorm,
-                           double (&berr_out)[kMaxGroup], bool* allconv, double* red) {
+                           double (&berr_out)[kMaxGroup], bool* allconv, double* red,
+                           bool fwd_done = false) {
     const int n = s.n, tid = threadIdx.x;
+    // fwd_done: wv already holds L^-1 P b (eliminated with the factorisation),
+    // so the first solve is the U solve only
     auto solve_t = [&]() {
         if (!ADJ) {
-            if (tri.Linv) lower_unit_solve_inv(tri, wv, gc);
+            if (fwd_done) fwd_done = false;
+            else if (tri.Linv) lower_unit_solve_inv(tri, wv, gc);
             else lower_unit_solve(tri, wv, gc);
             if (tri.Linv && tri.inv_u) upper_solve_inv(tri, wv, gc);
             else upper_solve(tri, wv, gc);
@@ -1420,6 +1431,7 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
 
     const cx<T>* LU;
     char* solvep;
+    int fused = 0;  // right-hand sides whose forward elimination the factorisation did
     if (lay.small) {
         cx<T>* A = reinterpret_cast<cx<T>*>(L.main);
         const size_t nn = (size_t)n * n;
@@ -1440,7 +1452,11 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
             return;
         }
         cx<T>* S = reinterpret_cast<cx<T>*>(kp.scratch) + (size_t)blockIdx.x * kp.slot_elems;
-        factor_blocked<T, MAXR>(s, w, w2, S, lay, L);
+        // right-hand sides eliminated with the factorisation (the first group of
+        // a forward sweep)
+        const int nr = (kp.flags & (kAdjoint | kFactorOut)) ? 0 : min(nrhs, lay.G);
+        factor_blocked<T, MAXR>(s, w, w2, S, lay, L, nr);
+        fused = nr;
         LU = S;
         solvep = L.main;
     }
@@ -1493,8 +1509,9 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
     tri.rmap = lay.small ? nullptr : L.rmap;
     tri.Dg = lay.small ? nullptr : reinterpret_cast<cx<T>*>(after);
     tri.bs = lay.jb;
-    tri.Linv = lay.small ? nullptr : LU + (size_t)n * n;
-    tri.Uinv = lay.small ? nullptr : LU + (size_t)n * n + (size_t)((n + lay.jb - 1) / lay.jb) * lay.jb * lay.jb;
+    tri.Linv = lay.small ? nullptr : LU + (size_t)n * (n + kMaxGroup);
+    tri.Uinv = lay.small ? nullptr
+                         : LU + (size_t)n * (n + kMaxGroup) + (size_t)((n + lay.jb - 1) / lay.jb) * lay.jb * lay.jb;
     tri.inv_u = refine ? 1 : 0;
     if (kp.flags & kAdjoint) {
         z64* xk = reinterpret_cast<z64*>(reinterpret_cast<char*>(rs) + align16((long long)n * G * 16));
@@ -1508,16 +1525,19 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
     for (int g0 = 0; g0 < nrhs; g0 += G) {
         const int gc = min(G, nrhs - g0);
         const double2* Fg = s.F + (size_t)g0 * s.ldf;
+        const bool fwd = g0 == 0 && fused == gc;
         for (int e = tid; e < n * gc; e += kThreads) {
             const int r = e % n, g = e / n;
-            wv[e] = to_t<T>(ld_z(Fg + L.rmap[r] + (size_t)g * s.ldf));
+            // forward-eliminated columns: y = L^-1 P b, row r at physical row rmap[r]
+            wv[e] = fwd ? LU[(size_t)L.rmap[r] + (size_t)(n + g) * n]
+                        : to_t<T>(ld_z(Fg + L.rmap[r] + (size_t)g * s.ldf));
         }
         __syncthreads();
         double be[kMaxGroup];
         bool conv;
         const int it = solve_group<T, false>(s, w, w2, tri, L.rmap, wv, xs, rs, gc, Fg, s.ldf,
                                              nullptr, refine, want_berr, kp.tol, kp.max_iter,
-                                             &anorm, be, &conv, L.red);
+                                             &anorm, be, &conv, L.red, fwd);
         if (refine && !conv) ok = false;
         if (it > it_max) it_max = it;
         for (int e = tid; e < n * gc; e += kThreads) {
