@@ -131,10 +131,6 @@ template <typename T>
 __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) { return cmul(a, crecip(b)); }
 template <typename T>
 __device__ __forceinline__ T cabs1(cx<T> a) { return fabs(a.re) + fabs(a.im); }
-template <typename T>
-__device__ __forceinline__ cx<T> cshfl(cx<T> z, int src) {
-    return cmk<T>(__shfl(z.re, src, 64), __shfl(z.im, src, 64));
-}
 // Broadcast from a wavefront-uniform lane: v_readlane into a scalar register
 // (no LDS crossbar traffic, a few cycles of latency).
 __device__ __forceinline__ float lane_read(float v, int l) {
@@ -149,11 +145,6 @@ __device__ __forceinline__ double lane_read(double v, int l) {
 template <typename T>
 __device__ __forceinline__ cx<T> cbcast(cx<T> z, int l) {
     return cmk<T>(lane_read(z.re, l), lane_read(z.im, l));
-}
-// Replace lane l of a register with the (uniform) value v.
-template <typename T>
-__device__ __forceinline__ cx<T> cput(cx<T> v, int l, cx<T> old) {
-    return ((int)(threadIdx.x & 63) == l) ? v : old;
 }
 template <typename T>
 __device__ __forceinline__ cx<T> to_t(z64 a) { return cmk<T>((T)a.re, (T)a.im); }
@@ -828,7 +819,7 @@ __device__ __forceinline__ void factor_blocked(const SysDesc& s, double w, doubl
 // ---- triangular solves on LDS vectors w (n x gc, ld n) -----------------------------------
 // LU: factors with logical row r at physical row rmap[r] (rmap == nullptr:
 // identity).  Dg: LDS staging for one kSolveBlock^2 diagonal block, or nullptr
-// when LU itself is in LDS.  Off-diagonal updates issue 8 loads per round.
+// when LU itself is in LDS.  Off-diagonal updates issue SolveRound loads per round.
 template <typename T>
 struct TriCtx {
     const cx<T>* LU;
