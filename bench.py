@@ -224,7 +224,7 @@ def other_workload(name, steps, warmup):
         X = torch.empty((c.freq.size, 16, 1024), dtype=torch.complex128, device=dev)
         info = torch.empty(c.freq.size, dtype=torch.int32, device=dev)
         be = torch.empty((c.freq.size, 16), dtype=torch.float64, device=dev)
-        stream = torch.cuda.current_stream()
+        stream = torch.cuda.Stream(device=dev)
         for mode in ("refined", "plain"):
             run = lambda: h.sweep_dev(0, fr, F, X, info, be if mode == "refined" else None,
                                       mode=mode, tol=TOL_REFINED, max_iter=10,

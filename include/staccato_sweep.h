@@ -198,6 +198,32 @@ int stc_back_transform(stc_handle h, int rom_id, int nfreq, const double* freq_h
                        stc_z64* H, int* info, int mode, double tol,
                        int max_iter, stc_stats* stats);
 
+/* ---- explicit matrices (batched getrf + getrs) ------------------------------
+ * Solve A_j X_j = B_j for `batch` given n x n matrices with LU and partial
+ * pivoting (zgetrf/zgetrs semantics; STC_REFINED_FP32 factors in complex64
+ * and refines to tol as for stc_sweep).  A_j at A + j*strideA (column-major,
+ * lda), B_j at B + j*strideB (n x nrhs, ldb; strideB = 0: one B for every j),
+ * X_j written at X + j*strideX (ldx).  info[j] and berr[j*nrhs + c]
+ * (nullable) as for stc_sweep.  n <= 1024.  Replaces the batched solves of
+ * the interface Jacobian, cublasZgetrfBatched + cublasZgetrsBatched in
+ * test/IJCSA/src/IJSA.cu:1201-1208 (there unpivoted: pivots NULL) and
+ * test/krylovMidsizeTest/gpu/concurrent/batchedLU/src/main.cu:254-259.
+ * Host memory; synchronous. */
+int stc_gesv_batched(stc_handle h, int n, int batch, const stc_z64* A, int lda,
+                     long long strideA, int nrhs, const stc_z64* B, int ldb,
+                     long long strideB, stc_z64* X, int ldx, long long strideX,
+                     int* info, double* berr, int mode, double tol, int max_iter,
+                     stc_stats* stats);
+
+/* Same with every array in device memory, enqueued on `stream` (NULL = handle
+ * stream): A_j at d_A + j*strideA with lda = n (strideA >= n*n), X_j at
+ * d_X + j*ldx*nrhs.  Does not synchronise. */
+int stc_gesv_batched_dev(stc_handle h, int n, int batch, const stc_z64* d_A,
+                         long long strideA, int nrhs, const stc_z64* d_B, int ldb,
+                         long long strideB, stc_z64* d_X, int ldx, int* d_info,
+                         double* d_berr, int mode, double tol, int max_iter,
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,7 +27,7 @@ EXPORTED = (
     "stc_rom_set", "stc_rom_set_io", "stc_rom_clear",
     "stc_sweep", "stc_sweep_dev", "stc_getrf",
     "stc_sweep_batch", "stc_plan_create", "stc_plan_execute", "stc_plan_destroy",
-    "stc_adjoint", "stc_back_transform",
+    "stc_adjoint", "stc_back_transform", "stc_gesv_batched", "stc_gesv_batched_dev",
 )
 
 
@@ -79,7 +79,7 @@ def load(path=None):
             f"{p} is missing: build it with staccato_amd.build.build_library() "
             "(hipcc --offload-arch=gfx950); the sweep has no CPU fallback")
     lib = ctypes.CDLL(p)
-    vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    vp, i, d, ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
     pi = ctypes.POINTER(ctypes.c_int)
     sig = {
         "stc_create": (i, [ctypes.POINTER(vp), i]),
@@ -99,6 +99,10 @@ def load(path=None):
         "stc_adjoint": (i, [vp, i, i, vp, i, vp, i, vp, i, vp, i, vp, i, d, i,
                             vp, vp, vp, vp]),
         "stc_back_transform": (i, [vp, i, i, vp, vp, i, vp, vp, vp, i, d, i, vp]),
+        "stc_gesv_batched": (i, [vp, i, i, vp, i, ll, i, vp, i, ll, vp, i, ll, vp, vp, i, d,
+                                 i, vp]),
+        "stc_gesv_batched_dev": (i, [vp, i, i, vp, ll, i, vp, i, ll, vp, i, vp, vp, i, d, i,
+                                     vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -63,7 +63,7 @@ struct stc_ctx {
     std::map<int, RomDev> roms;
     DevBuf scratch, counter;
     // staging for host-memory calls
-    DevBuf freq, F, X, info, berr, iters, fac, ipiv, sys, items, adj, c, phi, kd;
+    DevBuf freq, F, X, info, berr, iters, fac, ipiv, sys, items, adj, c, phi, kd, A;
 };
 
 struct stc_plan_s {
@@ -348,7 +348,7 @@ int stc_destroy(stc_handle h) {
         }
         DevBuf* bufs[] = {&h->scratch, &h->counter, &h->freq, &h->F, &h->X, &h->info, &h->berr,
                           &h->iters, &h->fac, &h->ipiv, &h->sys, &h->items, &h->adj, &h->c,
-                          &h->phi, &h->kd};
+                          &h->phi, &h->kd, &h->A};
         for (DevBuf* b : bufs) b->release();
         (void)hipStreamDestroy(h->stream);
     }
@@ -941,6 +941,116 @@ int stc_back_transform(stc_handle h, int rom_id, int nfreq, const double* freq_h
         stats->points = nfreq;
     }
     return status_from_info(info, nfreq, stats);
+}
+
+// ---- explicit matrices --------------------------------------------------------
+// The sweep kernel factors A = (1+i eta)K + i w D - w^2 M; with eta = 0, w = 0
+// (no frequency list) and one K block per frequency index it factors the given
+// matrices themselves (M aliases K and is multiplied by zero).
+int stc_gesv_batched_dev(stc_handle h, int n, int batch, const stc_z64* d_A, long long strideA,
+                         int nrhs, const stc_z64* d_B, int ldb, long long strideB, stc_z64* d_X,
+                         int ldx, int* d_info, double* d_berr, int mode, double tol,
+                         int max_iter, void* stream) {
+    if (!h || n < 1 || batch < 0 || nrhs < 1 || check_mode(mode)) return STC_EINVAL;
+    if (strideA < (long long)n * n || ldb < n || ldx < n ||
+        (strideB != 0 && strideB < (long long)ldb * nrhs))
+        return STC_EINVAL;
+    if (batch == 0) return STC_OK;
+    if (!d_A || !d_B || !d_X || !d_info) return STC_EINVAL;
+    DeviceGuard g(h->device);
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
+    KParams kp = base_params(mode, tol, max_iter);
+    std::vector<SysDesc> sys(1);
+    SysDesc& s = sys[0];
+    memset(&s, 0, sizeof(s));
+    s.n = n;
+    s.real_ops = 0;
+    s.has_D = 0;
+    s.eta = 0.0;
+    s.K = reinterpret_cast<const double*>(d_A);
+    s.M = s.K;
+    s.op_stride = 2 * strideA;
+    s.f_stride = strideB;
+    s.nrhs = nrhs;
+    s.nfreq = batch;
+    s.ldf = ldb;
+    s.ldx = ldx;
+    s.freq = nullptr;
+    s.F = reinterpret_cast<const double2*>(d_B);
+    s.X = reinterpret_cast<double2*>(d_X);
+    s.info = d_info;
+    s.berr = d_berr;
+    LaunchCfg cfg;
+    int rc = configure(h, sys, batch, mode == STC_REFINED_FP32, 0, 0, &cfg);
+    if (rc) return rc;
+    kp.one = sys[0];
+    kp.nitems = batch;
+    if (d_berr) kp.flags |= kWantBerr;
+    return enqueue(h, cfg, kp, mode == STC_REFINED_FP32, st);
+}
+
+int stc_gesv_batched(stc_handle h, int n, int batch, const stc_z64* A, int lda, long long strideA,
+                     int nrhs, const stc_z64* B, int ldb, long long strideB, stc_z64* X, int ldx,
+                     long long strideX, int* info, double* berr, int mode, double tol,
+                     int max_iter, stc_stats* stats) {
+    if (!h || n < 1 || batch < 0 || nrhs < 1 || check_mode(mode)) return STC_EINVAL;
+    if (lda < n || ldb < n || ldx < n || strideA < (long long)lda * n ||
+        (strideB != 0 && strideB < (long long)ldb * nrhs) || strideX < (long long)ldx * nrhs)
+        return STC_EINVAL;
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (batch == 0) return STC_OK;
+    if (!A || !B || !X || !info) return STC_EINVAL;
+    DeviceGuard g(h->device);
+    hipStream_t st = h->stream;
+    const size_t nn = (size_t)n * n, nb = strideB ? (size_t)batch : 1;
+    int rc;
+    if ((rc = h->A.ensure(sizeof(stc_z64) * nn * batch))) return rc;
+    if ((rc = h->F.ensure(sizeof(stc_z64) * (size_t)n * nrhs * nb))) return rc;
+    if ((rc = h->X.ensure(sizeof(stc_z64) * (size_t)n * nrhs * batch))) return rc;
+    if ((rc = h->info.ensure(sizeof(int) * batch))) return rc;
+    if (berr && (rc = h->berr.ensure(sizeof(double) * (size_t)batch * nrhs))) return rc;
+    // pack the matrices and right-hand sides (leading dimension n)
+    for (int j = 0; j < batch; ++j)
+        HIP_TRY(hipMemcpy2DAsync(h->A.as<stc_z64>() + (size_t)j * nn, sizeof(stc_z64) * n,
+                                 A + (size_t)j * strideA, sizeof(stc_z64) * lda,
+                                 sizeof(stc_z64) * n, n, hipMemcpyHostToDevice, st));
+    for (size_t j = 0; j < nb; ++j)
+        HIP_TRY(hipMemcpy2DAsync(h->F.as<stc_z64>() + j * n * nrhs, sizeof(stc_z64) * n,
+                                 B + j * strideB, sizeof(stc_z64) * ldb, sizeof(stc_z64) * n,
+                                 nrhs, hipMemcpyHostToDevice, st));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+    rc = stc_gesv_batched_dev(h, n, batch, h->A.as<stc_z64>(), (long long)nn, nrhs,
+                              h->F.as<stc_z64>(), n, strideB ? (long long)n * nrhs : 0,
+                              h->X.as<stc_z64>(), n, h->info.as<int>(),
+                              berr ? h->berr.as<double>() : nullptr, mode, tol, max_iter, st);
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    for (int j = 0; j < batch; ++j)
+        HIP_TRY(hipMemcpy2D(X + (size_t)j * strideX, sizeof(stc_z64) * ldx,
+                            h->X.as<stc_z64>() + (size_t)j * n * nrhs, sizeof(stc_z64) * n,
+                            sizeof(stc_z64) * n, nrhs, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(info, h->info.p, sizeof(int) * batch, hipMemcpyDeviceToHost));
+    if (berr)
+        HIP_TRY(hipMemcpy(berr, h->berr.p, sizeof(double) * batch * nrhs, hipMemcpyDeviceToHost));
+    if (stats) {
+        stats->seconds = ms * 1e-3;
+        stats->points = batch;
+        if (berr) {
+            double m = 0.0;
+            for (long long q = 0; q < (long long)batch * nrhs; ++q)
+                if (std::isfinite(berr[q])) m = std::max(m, berr[q]);
+            stats->max_backward_error = m;
+        }
+    }
+    return status_from_info(info, batch, stats);
 }
 
 }  // extern "C"

@@ -55,6 +55,10 @@ struct SysDesc {
     const double* kdiag; // p
     double* adj_out;     // nfreq blocks of (p + 2): J_w, dJ/deta_w, dJ/ds_k,w
     long long item_base; // first point index of this system in the queue
+    // explicit matrices: > 0 makes K, M and D advance by op_stride doubles and F by
+    // f_stride complex entries per frequency index (freq == nullptr: w = 0)
+    long long op_stride;
+    long long f_stride;
     // LDS plan chosen by the host (plan_layout) for this system
     int lay_small;
     int lay_jb;

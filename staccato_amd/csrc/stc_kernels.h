@@ -1409,7 +1409,7 @@ __device__ __forceinline__ void adjoint_point(const SysDesc& s, int fidx, double
 template <typename T, int MAXR>
 __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const KParams& kp, char* smem) {
     const int n = s.n, tid = threadIdx.x, nrhs = s.nrhs;
-    const double w = kTwoPi * s.freq[fidx], w2 = w * w;
+    const double w = s.freq ? kTwoPi * s.freq[fidx] : 0.0, w2 = w * w;
     Layout lay;
     lay.small = s.lay_small;
     lay.jb = s.lay_jb;
@@ -1586,7 +1586,14 @@ sweep_kernel(KParams kp) {
             sysi = __builtin_amdgcn_readfirstlane(it.x);
             fidx = __builtin_amdgcn_readfirstlane(it.y);
         }
-        const SysDesc s = kp.sys ? load_uniform(kp.sys + sysi) : load_uniform(&kp.one);
+        SysDesc s = kp.sys ? load_uniform(kp.sys + sysi) : load_uniform(&kp.one);
+        if (s.op_stride) {  // one explicit matrix per frequency index
+            const long long o = (long long)fidx * s.op_stride;
+            s.K += o;
+            s.M += o;
+            if (s.has_D) s.D += o;
+        }
+        if (s.f_stride) s.F += (long long)fidx * s.f_stride;
         process_point<T, MAXR>(s, fidx, kp, smem);
         __syncthreads();
     }

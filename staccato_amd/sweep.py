@@ -41,6 +41,18 @@ def _fortran_c128(a, ndim=2):
     return np.asfortranarray(a)
 
 
+def _stream_arg(stream):
+    """None selects the handle's own stream (NULL at the C ABI).  The null
+    stream (0, torch's default) cannot be named through the ABI, and work on
+    it is not ordered with the handle's non-blocking stream: refuse it."""
+    if stream is None:
+        return None
+    if int(stream) == 0:
+        raise ValueError("stream 0 (the null stream) cannot be passed: NULL selects the "
+                         "handle's stream; run the torch work on a torch.cuda.Stream")
+    return ctypes.c_void_p(int(stream))
+
+
 def _check(rc, what, soft=(0,)):
     if rc not in soft:
         raise StcError(rc, what)
@@ -56,6 +68,7 @@ class Handle:
         _check(self.lib.stc_create(ctypes.byref(h), int(device)), "stc_create")
         self.h = h
         self.n = {}
+        self.device = int(device)
 
     def close(self):
         if self.h:
@@ -122,7 +135,7 @@ class Handle:
         (nfreq, nrhs, n) complex128, info_t int32 (nfreq), berr_t float64 or None."""
         n = self.n[rom_id]
         nrhs = F_t.shape[0]
-        s = ctypes.c_void_p(stream) if stream else None
+        s = _stream_arg(stream)
         rc = self.lib.stc_sweep_dev(
             self.h, rom_id, freq_t.numel(), ctypes.c_void_p(freq_t.data_ptr()), nrhs,
             ctypes.c_void_p(F_t.data_ptr()), n, ctypes.c_void_p(X_t.data_ptr()), n,
@@ -206,6 +219,48 @@ class Handle:
         _check(rc, "stc_back_transform", soft=(0, 1, 2))
         return (Y.transpose(0, 2, 1), H.transpose(0, 2, 1) if transfer else None, info, st)
 
+    # ---- explicit matrices ------------------------------------------------------
+    def gesv_batched(self, A, B, mode="plain", tol=0.0, max_iter=0, berr=True):
+        """X[j] = A[j]^{-1} B[j] (or B shared: 2-D) for explicit matrices A
+        (batch, n, n), with pivoted LU as zgetrf/zgetrs.  Returns (X (batch,
+        n, nrhs), info, berr, stats)."""
+        A = np.asarray(A, dtype=np.complex128)
+        nb, n = A.shape[0], A.shape[1]
+        Ac = np.ascontiguousarray(A.transpose(0, 2, 1))      # column-major blocks
+        B = np.asarray(B, dtype=np.complex128)
+        if B.ndim == 1:
+            B = B[:, None]
+        shared = B.ndim == 2
+        Bc = np.ascontiguousarray(B.T if shared else B.transpose(0, 2, 1))
+        nrhs = Bc.shape[-2]
+        X = np.empty((nb, nrhs, n), dtype=np.complex128)
+        info = np.zeros(nb, dtype=np.int32)
+        be = np.empty((nb, nrhs), dtype=np.float64) if berr else None
+        st = Stats()
+        rc = self.lib.stc_gesv_batched(self.h, n, nb, _ptr(Ac), n, n * n, nrhs, _ptr(Bc), n,
+                                       0 if shared else n * nrhs, _ptr(X), n, n * nrhs,
+                                       _ptr(info), _ptr(be), _mode(mode), float(tol),
+                                       int(max_iter), ctypes.byref(st))
+        _check(rc, "stc_gesv_batched", soft=(0, 1, 2))
+        return X.transpose(0, 2, 1), info, be, st
+
+    def gesv_batched_dev(self, A_t, B_t, X_t, info_t, berr_t=None, mode="plain", tol=0.0,
+                         max_iter=0, stream=None):
+        """Device-resident batched solve on torch tensors (no copies, no
+        synchronisation).  A_t: (batch, n, n) complex128 holding each matrix
+        column-major (A[j] transposed in memory), B_t: (batch, nrhs, n)
+        column-major blocks, X_t: (batch, nrhs, n), info_t int32 (batch)."""
+        nb, n = A_t.shape[0], A_t.shape[1]
+        nrhs = B_t.shape[1]
+        s = _stream_arg(stream)
+        rc = self.lib.stc_gesv_batched_dev(
+            self.h, n, nb, ctypes.c_void_p(A_t.data_ptr()), n * n, nrhs,
+            ctypes.c_void_p(B_t.data_ptr()), n, n * nrhs, ctypes.c_void_p(X_t.data_ptr()), n,
+            ctypes.c_void_p(info_t.data_ptr()),
+            ctypes.c_void_p(berr_t.data_ptr()) if berr_t is not None else None,
+            _mode(mode), float(tol), int(max_iter), s)
+        _check(rc, "stc_gesv_batched_dev")
+
 
 class Plan:
     """A captured (hipGraph) persistent launch over device-resident systems."""
@@ -219,8 +274,7 @@ class Plan:
         self.p = p
 
     def execute(self, stream=None):
-        _check(self.handle.lib.stc_plan_execute(self.p, ctypes.c_void_p(stream) if stream else None),
-               "stc_plan_execute")
+        _check(self.handle.lib.stc_plan_execute(self.p, _stream_arg(stream)), "stc_plan_execute")
 
     def close(self):
         if self.p:
