@@ -232,6 +232,60 @@ def other_workload(name, steps, warmup):
             wall, _ = time_gpu(run, steps, warmup, stream)
             out[mode] = {"pts_per_s": c.freq.size * steps / wall,
                          "tflops": c.freq.size * work_per_point(1024, 16) * steps / wall / 1e12}
+    elif name == "ijcsa":
+        # interface-Jacobian coupling: a chain of three n=256 substructures
+        # (16 shared DoFs per joint, 4 loads, 8 outputs each), 4096 frequencies
+        from staccato_amd.coupling import Substructure, couple
+        rng = np.random.default_rng(7)
+        nfq, n, nj = 4096, 256, 16
+        freq = np.linspace(1.0, 2000.0, nfq)
+        subs = []
+        for k in range(3):
+            G = rng.standard_normal((n, n))
+            K = 1e4 * (G @ G.T / n + 0.5 * np.eye(n))
+            G = rng.standard_normal((n, n))
+            M = G @ G.T / n + 0.5 * np.eye(n)
+            h.rom_set(k, K, M, 1e-4 * K + 0.3 * M, 0.02)
+            iface = ([] if k == 0 else list(range(nj))) + ([] if k == 2 else list(range(n - nj, n)))
+            inner = [i for i in range(n) if i not in iface]
+            load = inner[:4] if k == 0 else []
+            f = rng.standard_normal(len(load)) + 0j
+            subs.append(Substructure(k, iface, load, inner[-8:], None, f))
+        Nu = sum(s.iface.size for s in subs)
+        dRdY, dRdU = np.zeros((Nu, Nu)), np.zeros((Nu, Nu))
+        offs = np.cumsum([0] + [s.iface.size for s in subs])
+        row = 0
+        for k in range(2):
+            a_cols = offs[k] + subs[k].iface.size - nj + np.arange(nj)
+            b_cols = offs[k + 1] + np.arange(nj)
+            for q in range(nj):
+                dRdY[row, a_cols[q]], dRdY[row, b_cols[q]] = 1.0, -1.0
+                dRdU[row + nj, a_cols[q]], dRdU[row + nj, b_cols[q]] = 1.0, 1.0
+                row += 1
+            row += nj
+        # the interface-Jacobian solves alone, device resident (Nu = 64)
+        Js = torch.as_tensor(rng.standard_normal((nfq, Nu, Nu)) + 8.0 * np.eye(Nu) + 0j, device=dev)
+        rs = torch.as_tensor(rng.standard_normal((nfq, 1, Nu)) + 0j, device=dev)
+        ds = torch.empty_like(rs)
+        inf = torch.empty(nfq, dtype=torch.int32, device=dev)
+        stream = torch.cuda.Stream(device=dev)
+        for mode in ("refined", "plain"):
+            tol = TOL_REFINED if mode == "refined" else 0.0
+            for _ in range(warmup):
+                couple(h, subs, dRdY, dRdU, freq, mode=mode, tol=tol)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                res = couple(h, subs, dRdY, dRdU, freq, mode=mode, tol=tol)
+            dt = (time.perf_counter() - t0) / steps
+            bad = int((res["info_J"] != 0).sum()) + sum(int((i != 0).sum()) for i in res["info_sub"])
+            run = lambda: h.gesv_batched_dev(Js, rs, ds, inf, mode=mode, tol=tol,
+                                             stream=stream.cuda_stream)
+            wall, ev = time_gpu(run, steps, warmup, stream)
+            out[mode] = {"coupled_pts_per_s_incl_host_copies": nfq / dt, "points_failed": bad,
+                         "interface_solve_pts_per_s": nfq * steps / wall,
+                         "interface_solve_kernel_ms": 1e3 * ev / steps}
+        out["config"] = {"substructures": 3, "n": n, "shared_dofs_per_joint": nj, "Nu": int(Nu),
+                         "nfreq": nfq, "loads": 4, "outputs_per_substructure": 8}
     h.close()
     print(json.dumps(out), flush=True)
 
@@ -245,7 +299,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="both", choices=["both", "refined", "plain"])
     ap.add_argument("--workload", default="uniform",
-                    choices=["uniform", "uneven", "adjoint", "dense1024"],
+                    choices=["uniform", "uneven", "adjoint", "dense1024", "ijcsa"],
                     help="uniform = the headline n=256 sweep; the others print their own line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (a kernel of known "

@@ -47,6 +47,20 @@ def test_strerror_and_invalid_arguments():
     assert lib.stc_device_ok(None) == 0
     assert lib.stc_rom_set(None, 0, 4, None, None, None, 4, 0.0) == _lib.STC_EINVAL
     assert lib.stc_plan_destroy(None) == _lib.STC_EINVAL
+    assert lib.stc_gesv_batched(None, 4, 1, None, 4, 16, 1, None, 4, 4, None, 4, 4, None, None,
+                                0, 0.0, 0, None) == _lib.STC_EINVAL
+    assert lib.stc_gesv_batched_dev(None, 4, 1, None, 16, 1, None, 4, 4, None, 4, None, None,
+                                    0, 0.0, 0, None) == _lib.STC_EINVAL
+
+
+def test_null_stream_is_refused():
+    """NULL at the ABI selects the handle's stream, so the Python wrappers
+    refuse stream 0 (the null stream) instead of silently changing streams."""
+    from staccato_amd.sweep import _stream_arg
+    assert _stream_arg(None) is None
+    assert _stream_arg(0x1234).value == 0x1234
+    with pytest.raises(ValueError):
+        _stream_arg(0)
 
 
 def test_missing_library_fails_loudly(tmp_path):
