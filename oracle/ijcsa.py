@@ -5,8 +5,9 @@ package.  Follows test/IJCSA/src/IJSA.cu for simulation models, one Newton
 step of the interface residual per frequency, in complex128 numpy:
 
   per substructure s (IJSA.cu:830-917): A_s(w) = (1+i eta)K_s + i w D_s - w^2 M_s,
-      H_s = C_s A_s^{-1} B_s with B_s = [E_iface | E_load] and
-      C_s = [E_iface ; E_internal]  (zgetrf + two zgetrs + four GEMMs)
+      H_s = C_s A_s^{-1} B_s with B_s = [B_iface | B_load] and
+      C_s = [C_iface ; C_internal] (zgetrf + two zgetrs + four GEMMs; the
+      matrices are given, or unit-vector selections of index lists)
   initial interface solution (IJSA.cu:920-946): Y_s = H_ii u_s + H_il f_s
   residual (IJSA.cu:948-968):  r = sum_s dRdY_s Y_s + dRdU_s u_s
   Jacobian (IJSA.cu:523-527, 970-981): J = dRdU + sum_s dRdY_s H_ii (column
@@ -36,27 +37,41 @@ def assemble(K, M, D, eta, w):
     return A
 
 
-def frf_blocks(K, M, D, eta, w, iface, load, internal):
-    """(H_ii, H_il, H_ni, H_nl) of one substructure at angular frequency w."""
-    n = K.shape[0]
-    E = np.eye(n)
-    cols = list(iface) + list(load)
-    X = np.linalg.solve(assemble(K, M, D, eta, w), E[:, cols].astype(np.complex128))
-    ni = len(iface)
-    Hi = X[list(iface), :]
-    Hn = X[list(internal), :] if len(internal) else np.zeros((0, len(cols)), np.complex128)
+def io_matrices(s):
+    """(B_iface, B_load, C_iface, C_internal) of a substructure dict: the
+    matrices when given (a ROM's B_R / C_R blocks, IJSA.cu:869-917 multiplies
+    by them), else unit-vector selections of the index lists."""
+    n = s["K"].shape[0]
+    if "B_iface" in s:
+        return (np.asarray(s["B_iface"], np.complex128), np.asarray(s["B_load"], np.complex128),
+                np.asarray(s["C_iface"], np.complex128), np.asarray(s["C_internal"], np.complex128))
+    E = np.eye(n, dtype=np.complex128)
+    return (E[:, list(s["iface"])], E[:, list(s["load"])], E[list(s["iface"]), :],
+            E[list(s["internal"]), :])
+
+
+def frf_blocks(s, w):
+    """(H_ii, H_il, H_ni, H_nl) of one substructure at angular frequency w:
+    X = A^{-1} [B_iface | B_load] (zgetrf + zgetrs, IJSA.cu:836-866), then
+    C_iface X and C_internal X (IJSA.cu:869-917)."""
+    Bi, Bl, Ci, Cn = io_matrices(s)
+    ni = Bi.shape[1]
+    X = np.linalg.solve(assemble(s["K"], s["M"], s.get("D"), s["eta"], w), np.hstack([Bi, Bl]))
+    Hi, Hn = Ci @ X, Cn @ X
     return Hi[:, :ni], Hi[:, ni:], Hn[:, :ni], Hn[:, ni:]
 
 
 def couple(subs, dRdY, dRdU, freq):
-    """subs: list of dicts with K, M, D (or None), eta, iface, load, internal,
-    u0 (ni), f (nl).  Returns (u (nfreq, Nu), Y (nfreq, Nu), [Y_int_s (nfreq, nn_s)])."""
-    Nu = sum(len(s["iface"]) for s in subs)
+    """subs: list of dicts with K, M, D (or None), eta, u0 (ni), f (nl) and
+    either index lists iface, load, internal or the matrices B_iface (n x ni),
+    B_load (n x nl), C_iface (ni x n), C_internal (nn x n).  Returns
+    (u (nfreq, Nu), Y (nfreq, Nu), [Y_int_s (nfreq, nn_s)])."""
+    Nu = sum(len(s["u0"]) for s in subs)
     assert dRdY.shape == (Nu, Nu) and dRdU.shape == (Nu, Nu)
     nf = len(freq)
     U = np.zeros((nf, Nu), np.complex128)
     Y = np.zeros((nf, Nu), np.complex128)
-    Yint = [np.zeros((nf, len(s["internal"])), np.complex128) for s in subs]
+    Yint = [np.zeros((nf, io_matrices(s)[3].shape[0]), np.complex128) for s in subs]
     for j, fz in enumerate(freq):
         w = 2.0 * np.pi * fz
         H, u, Ys = [], [], []
@@ -64,9 +79,8 @@ def couple(subs, dRdY, dRdU, freq):
         r = np.zeros(Nu, np.complex128)
         off = 0
         for s in subs:
-            ni = len(s["iface"])
-            Hb = frf_blocks(s["K"], s["M"], s.get("D"), s["eta"], w, s["iface"], s["load"],
-                            s["internal"])
+            ni = len(s["u0"])
+            Hb = frf_blocks(s, w)
             us = np.array(s["u0"], dtype=np.complex128)
             fs = np.array(s["f"], dtype=np.complex128)
             ys = Hb[0] @ us + (Hb[1] @ fs if fs.size else 0)
@@ -79,13 +93,13 @@ def couple(subs, dRdY, dRdU, freq):
         delta = np.linalg.solve(J, r)
         off = 0
         for k, s in enumerate(subs):
-            ni = len(s["iface"])
+            ni = len(s["u0"])
             us = u[k] - delta[off:off + ni]
             fs = np.array(s["f"], dtype=np.complex128)
             Hii, Hil, Hni, Hnl = H[k]
             U[j, off:off + ni] = us
             Y[j, off:off + ni] = Hii @ us + (Hil @ fs if fs.size else 0)
-            if len(s["internal"]):
+            if Hni.shape[0]:
                 Yint[k][j] = Hni @ us + (Hnl @ fs if fs.size else 0)
             off += ni
     return U, Y, Yint

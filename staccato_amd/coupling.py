@@ -1,11 +1,12 @@
 """Interface-Jacobian coupled substructure analysis (IJCSA) on the GPU.
 
 Mirrors the simulation-model path of test/IJCSA/src/IJSA.cu, one Newton step
-of the interface residual per frequency:
+of the interface residual per frequency, with every array device resident:
 
 * FRFs of every substructure at its interface, load and internal-output DoFs,
-  H_s(w) = C_s A_s(w)^{-1} B_s (IJSA.cu:830-917), from the sweep kernel
-  (Handle.back_transform with B_s = [E_iface | E_load], C_s = [E_iface ; E_internal]);
+  H_s(w) = C_s A_s(w)^{-1} B_s (IJSA.cu:830-917): one sweep of the ROM
+  (Handle.sweep_dev) with right-hand sides [B_iface u0 + B_load f | B_iface |
+  B_load], then C_iface X and C_internal X as batched device GEMMs;
 * initial interface solution Y_s = H_ii u_s + H_il f_s (IJSA.cu:920-946),
   residual r = sum_s dRdY_s Y_s + dRdU_s u_s (IJSA.cu:948-968) and interface
   Jacobian J = dRdU + sum_s dRdY_s H_ii (IJSA.cu:523-527, 970-981): batched
@@ -26,55 +27,66 @@ from .sweep import StcError
 
 
 class Substructure:
-    """A simulation model: the id of a ROM already set on the handle
-    (rom_set), 0-based local indices of its interface, load and internal
-    output DoFs, its initial interface forces u0 and its load values f.
-    couple() sets the ROM's input/output maps (rom_set_io)."""
+    """A simulation model: the id of a ROM already set on the handle (rom_set),
+    its input matrices B_iface (n x ni) and B_load (n x nl), its output matrices
+    C_iface (ni x n) and C_internal (nn x n) (the interface/load columns of the
+    ROM's B_R and the interface/internal rows of its C_R, IJSA.cu:869-917), its
+    initial interface forces u0 (ni) and its load values f (nl)."""
 
-    def __init__(self, rom_id, iface, load=(), internal=(), u0=None, f=None):
+    def __init__(self, rom_id, B_iface, C_iface, B_load=None, C_internal=None, u0=None, f=None):
         self.rom_id = rom_id
-        self.iface = np.asarray(iface, dtype=np.int64)
-        self.load = np.asarray(load, dtype=np.int64)
-        self.internal = np.asarray(internal, dtype=np.int64)
-        ni, nl = self.iface.size, self.load.size
+        self.B_iface = np.asarray(B_iface, np.complex128)
+        n, ni = self.B_iface.shape
+        self.C_iface = np.asarray(C_iface, np.complex128)
+        self.B_load = (np.zeros((n, 0), np.complex128) if B_load is None
+                       else np.asarray(B_load, np.complex128).reshape(n, -1))
+        self.C_internal = (np.zeros((0, n), np.complex128) if C_internal is None
+                           else np.asarray(C_internal, np.complex128).reshape(-1, n))
+        nl = self.B_load.shape[1]
         self.u0 = np.zeros(ni, np.complex128) if u0 is None else np.asarray(u0, np.complex128)
         self.f = np.zeros(nl, np.complex128) if f is None else np.asarray(f, np.complex128)
-        if ni < 1 or self.u0.shape != (ni,) or self.f.shape != (nl,):
-            raise ValueError("substructure needs >= 1 interface DoF, u0 (ni,) and f (nl,)")
+        if ni < 1 or self.C_iface.shape != (ni, n) or self.u0.shape != (ni,) \
+                or self.f.shape != (nl,):
+            raise ValueError("substructure needs B_iface (n x ni), C_iface (ni x n), ni >= 1, "
+                             "u0 (ni,) and f (nl,)")
 
+    @classmethod
+    def from_dofs(cls, rom_id, n, iface, load=(), internal=(), u0=None, f=None):
+        """Interface, load and output DoFs given as 0-based indices of the ROM."""
+        E = np.eye(n, dtype=np.complex128)
+        for idx in (iface, load, internal):
+            if len(idx) and (min(idx) < 0 or max(idx) >= n):
+                raise ValueError("DoF index outside the ROM")
+        return cls(rom_id, E[:, list(iface)], E[list(iface), :], E[:, list(load)],
+                   E[list(internal), :], u0, f)
 
-def _io_maps(n, s):
-    ni, nl, nn = s.iface.size, s.load.size, s.internal.size
-    for idx in (s.iface, s.load, s.internal):
-        if idx.size and (idx.min() < 0 or idx.max() >= n):
-            raise ValueError("DoF index outside the ROM")
-    B = np.zeros((n, ni + nl), np.complex128)
-    B[s.iface, np.arange(ni)] = 1.0
-    B[s.load, ni + np.arange(nl)] = 1.0
-    C = np.zeros((ni + nn, n), np.complex128)
-    C[np.arange(ni), s.iface] = 1.0
-    C[ni + np.arange(nn), s.internal] = 1.0
-    return B, C
+    @property
+    def n(self):
+        return self.B_iface.shape[0]
 
 
 def couple(h, subs, dRdY, dRdU, freq, mode="plain", tol=0.0, max_iter=0):
     """Coupled responses at the frequencies freq (Hz).  Returns a dict with
     u (nfreq, Nu) interface forces, Y (nfreq, Nu) interface responses,
     Y_internal [per substructure (nfreq, nn_s)], info_sub [per substructure
-    (nfreq,)] and info_J (nfreq,) (0 = ok, k > 0 = zero pivot k)."""
+    (nfreq,)] and info_J (nfreq,) (0 = ok, k > 0 = zero pivot k, -1 = not
+    refined to tol)."""
     import torch
 
     if not torch.cuda.is_available():
         raise RuntimeError("couple() runs on the GPU; no device is visible")
     dev = torch.device("cuda", h.device)
     freq = np.ascontiguousarray(freq, dtype=np.float64)
-    Nu = int(sum(s.iface.size for s in subs))
+    Nu = int(sum(s.u0.size for s in subs))
     dRdY = np.asarray(dRdY, np.complex128)
     dRdU = np.asarray(dRdU, np.complex128)
     if dRdY.shape != (Nu, Nu) or dRdU.shape != (Nu, Nu):
         raise ValueError(f"dRdY and dRdU must be {Nu} x {Nu}")
+    for s in subs:
+        if h.n.get(s.rom_id) != s.n:
+            raise ValueError(f"ROM {s.rom_id} is not set with n = {s.n}")
     # all device work of the step on one stream of its own, which the library's
-    # kernel is enqueued on too (the null stream cannot be named through the ABI)
+    # kernels are enqueued on too (the null stream cannot be named through the ABI)
     side = torch.cuda.Stream(device=dev)
     side.wait_stream(torch.cuda.current_stream(dev))
     with torch.cuda.stream(side):
@@ -87,60 +99,65 @@ def _couple_on(h, subs, dRdY, dRdU, freq, mode, tol, max_iter, dev, side):
     c128 = torch.complex128
     nf = freq.size
     Nu = dRdY.shape[0]
+    freq_t = torch.as_tensor(freq, device=dev)
     dY_t = torch.as_tensor(dRdY, device=dev)
     dU_t = torch.as_tensor(dRdU, device=dev)
     J = dU_t.expand(nf, Nu, Nu).clone()
     r = torch.zeros((nf, Nu, 1), dtype=c128, device=dev)
-    parts, info_sub = [], []
+    parts, infos = [], []
     off = 0
     for s in subs:
-        n = h.n[s.rom_id]
-        ni, nl, nn = s.iface.size, s.load.size, s.internal.size
-        B, C = _io_maps(n, s)
-        h.rom_set_io(s.rom_id, B, C)
-        load = np.concatenate([s.u0, s.f])[:, None]
-        Y0, H, info, _ = h.back_transform(s.rom_id, freq, load, ni + nl, ni + nn, transfer=True,
-                                          mode=mode, tol=tol, max_iter=max_iter)
-        info_sub.append(info)
-        H_t = torch.as_tensor(np.ascontiguousarray(H), device=dev)  # (nf, ni+nn, ni+nl)
-        y0 = torch.as_tensor(np.ascontiguousarray(Y0[:, :ni, :]), device=dev)
+        ni, nl = s.u0.size, s.f.size
+        # right-hand sides (column-major n x ncol, stored as (ncol, n)): the
+        # load B_iface u0 + B_load f, then the columns of B_iface and B_load
+        rhs = np.hstack([(s.B_iface @ s.u0 + s.B_load @ s.f)[:, None], s.B_iface, s.B_load])
+        F_t = torch.as_tensor(np.ascontiguousarray(rhs.T), device=dev)
+        X_t = torch.empty((nf, rhs.shape[1], s.n), dtype=c128, device=dev)
+        info_t = torch.empty(nf, dtype=torch.int32, device=dev)
+        h.sweep_dev(s.rom_id, freq_t, F_t, X_t, info_t, None, mode=mode, tol=tol,
+                    max_iter=max_iter, stream=side.cuda_stream)
+        infos.append(info_t)
+        # H = C X as (X^T C^T)^T: both operands of the batched product contiguous
+        CiT = torch.as_tensor(np.ascontiguousarray(s.C_iface.T), device=dev)
+        Hi = torch.matmul(X_t, CiT).transpose(1, 2)         # (nf, ni, ncol)
+        Hn = None
+        if s.C_internal.shape[0]:
+            CnT = torch.as_tensor(np.ascontiguousarray(s.C_internal.T), device=dev)
+            Hn = torch.matmul(X_t, CnT).transpose(1, 2)     # (nf, nn, ncol)
         u0 = torch.as_tensor(s.u0[:, None], device=dev)
         f = torch.as_tensor(s.f[:, None], device=dev)
+        y0 = Hi[:, :, :1].contiguous()                      # H_ii u0 + H_il f
         dYs = dY_t[:, off:off + ni].contiguous()
         dUs = dU_t[:, off:off + ni].contiguous()
         r += torch.matmul(dYs, y0) + torch.matmul(dUs, u0)
-        J[:, :, off:off + ni] += torch.matmul(dYs, H_t[:, :ni, :ni].contiguous())
-        parts.append((H_t, u0, f, ni, nl, nn, off))
+        J[:, :, off:off + ni] += torch.matmul(dYs, Hi[:, :, 1:1 + ni].contiguous())
+        parts.append((Hi, Hn, u0, f, ni, nl, off))
         off += ni
-    # delta = J^{-1} r on the handle's kernel, ordered after the GEMMs above
+    # delta = J^{-1} r by the LU kernel, on the same stream as the GEMMs above
     Jc = J.transpose(1, 2).contiguous()
     rc = r.transpose(1, 2).contiguous()
     delta = torch.empty_like(rc)
-    info_J = torch.zeros(nf, dtype=torch.int32, device=dev)
+    info_J = torch.empty(nf, dtype=torch.int32, device=dev)
     h.gesv_batched_dev(Jc, rc, delta, info_J, mode=mode, tol=tol, max_iter=max_iter,
                        stream=side.cuda_stream)
-    delta = delta.transpose(1, 2)  # (nf, Nu, 1)
+    delta = delta.transpose(1, 2)                           # (nf, Nu, 1)
     U = torch.empty((nf, Nu, 1), dtype=c128, device=dev)
     Y = torch.empty((nf, Nu, 1), dtype=c128, device=dev)
     Yint = []
-    for H_t, u0, f, ni, nl, nn, off in parts:
+    for Hi, Hn, u0, f, ni, nl, off in parts:
         u = u0 - delta[:, off:off + ni, :]
         U[:, off:off + ni] = u
-        y = torch.matmul(H_t[:, :ni, :ni].contiguous(), u)
-        if nl:
-            y = y + torch.matmul(H_t[:, :ni, ni:].contiguous(), f)
-        Y[:, off:off + ni] = y
-        if nn:
-            yi = torch.matmul(H_t[:, ni:, :ni].contiguous(), u)
-            if nl:
-                yi = yi + torch.matmul(H_t[:, ni:, ni:].contiguous(), f)
-            Yint.append(yi[:, :, 0].cpu().numpy())
+        # [H_ii | H_il] [u; f] and [H_ni | H_nl] [u; f]
+        uf = torch.cat([u, f.expand(nf, nl, 1)], dim=1)
+        Y[:, off:off + ni] = torch.matmul(Hi[:, :, 1:].contiguous(), uf)
+        if Hn is not None:
+            Yint.append(torch.matmul(Hn[:, :, 1:].contiguous(), uf)[:, :, 0])
         else:
-            Yint.append(np.zeros((nf, 0), np.complex128))
+            Yint.append(torch.zeros((nf, 0), dtype=c128, device=dev))
     side.synchronize()
-    out = {"u": U[:, :, 0].cpu().numpy(), "Y": Y[:, :, 0].cpu().numpy(), "Y_internal": Yint,
-           "info_sub": info_sub, "info_J": info_J.cpu().numpy()}
-    return out
+    return {"u": U[:, :, 0].cpu().numpy(), "Y": Y[:, :, 0].cpu().numpy(),
+            "Y_internal": [y.cpu().numpy() for y in Yint],
+            "info_sub": [i.cpu().numpy() for i in infos], "info_J": info_J.cpu().numpy()}
 
 
 __all__ = ["Substructure", "couple", "StcError"]

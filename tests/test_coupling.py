@@ -113,8 +113,7 @@ def _couple_without_dRdU_in_J(subs, dRdY, dRdU, freq):
         off, H = 0, []
         for s in subs:
             ni = len(s["iface"])
-            Hb = ijcsa.frf_blocks(s["K"], s["M"], s["D"], s["eta"], w, s["iface"], s["load"],
-                                  s["internal"])
+            Hb = ijcsa.frf_blocks(s, w)
             ys = Hb[0] @ s["u0"] + (Hb[1] @ s["f"] if len(s["f"]) else 0)
             r += dRdY[:, off:off + ni] @ ys + dRdU[:, off:off + ni] @ s["u0"]
             J[:, off:off + ni] += dRdY[:, off:off + ni] @ Hb[0]
@@ -222,7 +221,8 @@ def test_gpu_coupling_matches_oracle_and_monolithic(handle, mode, sizes, ni):
     for k, s in enumerate(subs):
         rid = 40 + k
         handle.rom_set(rid, s["K"], s["M"], s["D"], s["eta"])
-        gsubs.append(Substructure(rid, s["iface"], s["load"], s["internal"], s["u0"], s["f"]))
+        gsubs.append(Substructure.from_dofs(rid, s["K"].shape[0], s["iface"], s["load"],
+                                            s["internal"], s["u0"], s["f"]))
     tol = 1e-12 if mode == "refined" else 0.0
     out = couple(handle, gsubs, dRdY, dRdU, freq, mode=mode, tol=tol)
     assert (out["info_J"] == 0).all() and all((i == 0).all() for i in out["info_sub"])
@@ -237,3 +237,38 @@ def test_gpu_coupling_matches_oracle_and_monolithic(handle, mode, sizes, ni):
     assert np.abs(out["u"] - U).max() <= gate_u * np.abs(U).max()
     Xm = monolithic(glob, subs[0]["eta"], freq, F)
     check_against_monolithic(subs, gmaps, out["Y"], out["Y_internal"], Xm, gate)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["plain", "refined"])
+def test_gpu_coupling_dense_io_matches_oracle(handle, mode):
+    """General B/C blocks (a ROM's B_R / C_R, not DoF selections), nonzero
+    initial interface forces: the GPU step against the oracle's."""
+    from staccato_amd.coupling import couple
+    subs, glob, gmaps, dRdY, dRdU, F = chain_case((30, 24), 3, seed=21)
+    rng = np.random.default_rng(22)
+    freq = np.linspace(5.0, 700.0, 29)
+    gsubs = []
+    for k, s in enumerate(subs):
+        n = s["K"].shape[0]
+        E = np.eye(n)
+        cn = lambda *shape: 0.1 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+        s["B_iface"] = E[:, s["iface"]] + cn(n, len(s["iface"]))
+        s["C_iface"] = E[s["iface"], :] + cn(len(s["iface"]), n)
+        s["B_load"] = E[:, s["load"]] + cn(n, len(s["load"]))
+        s["C_internal"] = cn(5, n)
+        s["u0"] = cn(len(s["iface"]))
+        rid = 50 + k
+        handle.rom_set(rid, s["K"], s["M"], s["D"], s["eta"])
+        gsubs.append(Substructure(rid, s["B_iface"], s["C_iface"], s["B_load"], s["C_internal"],
+                                  s["u0"], s["f"]))
+    print(f"coupling dense io mode={mode}", flush=True)
+    out = couple(handle, gsubs, dRdY, dRdU, freq, mode=mode, tol=1e-12 if mode == "refined" else 0)
+    U, Y, Yint = ijcsa.couple(subs, dRdY, dRdU, freq)
+    gate = 1e-10 if mode == "plain" else 1e-8
+    gate_u = 1e-9 if mode == "plain" else 1e-6
+    assert (out["info_J"] == 0).all() and all((i == 0).all() for i in out["info_sub"])
+    assert np.abs(out["Y"] - Y).max() <= gate * np.abs(Y).max()
+    assert np.abs(out["u"] - U).max() <= gate_u * np.abs(U).max()
+    for a, b in zip(out["Y_internal"], Yint):
+        assert np.abs(a - b).max() <= gate * np.abs(b).max()
