@@ -61,17 +61,26 @@ def frf_blocks(s, w):
     return Hi[:, :ni], Hi[:, ni:], Hn[:, :ni], Hn[:, ni:]
 
 
+def measured_blocks(s, j):
+    """Experimental model (IJSA.cu:1011-1187): its FRF blocks at frequency j
+    are given, H_ii (nf, ni, ni), H_il (nf, ni, nl), H_ni (nf, nn, ni),
+    H_nl (nf, nn, nl)."""
+    return s["H_ii"][j], s["H_il"][j], s["H_ni"][j], s["H_nl"][j]
+
+
 def couple(subs, dRdY, dRdU, freq):
-    """subs: list of dicts with K, M, D (or None), eta, u0 (ni), f (nl) and
-    either index lists iface, load, internal or the matrices B_iface (n x ni),
-    B_load (n x nl), C_iface (ni x n), C_internal (nn x n).  Returns
+    """subs: list of dicts with u0 (ni), f (nl) and either a simulation model
+    (K, M, D (or None), eta, and index lists iface, load, internal or the
+    matrices B_iface (n x ni), B_load (n x nl), C_iface (ni x n), C_internal
+    (nn x n)) or an experimental one (H_ii, H_il, H_ni, H_nl).  Returns
     (u (nfreq, Nu), Y (nfreq, Nu), [Y_int_s (nfreq, nn_s)])."""
     Nu = sum(len(s["u0"]) for s in subs)
     assert dRdY.shape == (Nu, Nu) and dRdU.shape == (Nu, Nu)
     nf = len(freq)
     U = np.zeros((nf, Nu), np.complex128)
     Y = np.zeros((nf, Nu), np.complex128)
-    Yint = [np.zeros((nf, io_matrices(s)[3].shape[0]), np.complex128) for s in subs]
+    Yint = [np.zeros((nf, s["H_ni"].shape[1] if "H_ii" in s else io_matrices(s)[3].shape[0]),
+                     np.complex128) for s in subs]
     for j, fz in enumerate(freq):
         w = 2.0 * np.pi * fz
         H, u, Ys = [], [], []
@@ -80,7 +89,7 @@ def couple(subs, dRdY, dRdU, freq):
         off = 0
         for s in subs:
             ni = len(s["u0"])
-            Hb = frf_blocks(s, w)
+            Hb = measured_blocks(s, j) if "H_ii" in s else frf_blocks(s, w)
             us = np.array(s["u0"], dtype=np.complex128)
             fs = np.array(s["f"], dtype=np.complex128)
             ys = Hb[0] @ us + (Hb[1] @ fs if fs.size else 0)

@@ -65,6 +65,32 @@ class Substructure:
         return self.B_iface.shape[0]
 
 
+class MeasuredSubstructure:
+    """An experimental model (IJSA.cu:1011-1187): its FRFs at the coupling
+    frequencies are given instead of a ROM, H_ii (nf, ni, ni) interface to
+    interface, H_il (nf, ni, nl) load to interface, H_ni (nf, nn, ni) and
+    H_nl (nf, nn, nl) to its internal outputs; u0 (ni) and f (nl) as for
+    Substructure."""
+
+    def __init__(self, H_ii, H_il=None, H_ni=None, H_nl=None, u0=None, f=None):
+        self.H_ii = np.asarray(H_ii, np.complex128)
+        nf, ni = self.H_ii.shape[0], self.H_ii.shape[1]
+        self.H_il = (np.zeros((nf, ni, 0), np.complex128) if H_il is None
+                     else np.asarray(H_il, np.complex128))
+        nl = self.H_il.shape[2]
+        self.H_ni = (np.zeros((nf, 0, ni), np.complex128) if H_ni is None
+                     else np.asarray(H_ni, np.complex128))
+        nn = self.H_ni.shape[1]
+        self.H_nl = (np.zeros((nf, nn, nl), np.complex128) if H_nl is None
+                     else np.asarray(H_nl, np.complex128))
+        self.u0 = np.zeros(ni, np.complex128) if u0 is None else np.asarray(u0, np.complex128)
+        self.f = np.zeros(nl, np.complex128) if f is None else np.asarray(f, np.complex128)
+        if self.H_ii.shape != (nf, ni, ni) or self.H_il.shape != (nf, ni, nl) or \
+                self.H_ni.shape != (nf, nn, ni) or self.H_nl.shape != (nf, nn, nl) or \
+                self.u0.shape != (ni,) or self.f.shape != (nl,):
+            raise ValueError("measured FRF blocks, u0 and f do not agree in shape")
+
+
 def couple(h, subs, dRdY, dRdU, freq, mode="plain", tol=0.0, max_iter=0):
     """Coupled responses at the frequencies freq (Hz).  Returns a dict with
     u (nfreq, Nu) interface forces, Y (nfreq, Nu) interface responses,
@@ -83,7 +109,10 @@ def couple(h, subs, dRdY, dRdU, freq, mode="plain", tol=0.0, max_iter=0):
     if dRdY.shape != (Nu, Nu) or dRdU.shape != (Nu, Nu):
         raise ValueError(f"dRdY and dRdU must be {Nu} x {Nu}")
     for s in subs:
-        if h.n.get(s.rom_id) != s.n:
+        if isinstance(s, MeasuredSubstructure):
+            if s.H_ii.shape[0] != freq.size:
+                raise ValueError("measured FRFs must be given at every coupling frequency")
+        elif h.n.get(s.rom_id) != s.n:
             raise ValueError(f"ROM {s.rom_id} is not set with n = {s.n}")
     # all device work of the step on one stream of its own, which the library's
     # kernels are enqueued on too (the null stream cannot be named through the ABI)
@@ -108,24 +137,23 @@ def _couple_on(h, subs, dRdY, dRdU, freq, mode, tol, max_iter, dev, side):
     off = 0
     for s in subs:
         ni, nl = s.u0.size, s.f.size
-        # right-hand sides (column-major n x ncol, stored as (ncol, n)): the
-        # load B_iface u0 + B_load f, then the columns of B_iface and B_load
-        rhs = np.hstack([(s.B_iface @ s.u0 + s.B_load @ s.f)[:, None], s.B_iface, s.B_load])
-        F_t = torch.as_tensor(np.ascontiguousarray(rhs.T), device=dev)
-        X_t = torch.empty((nf, rhs.shape[1], s.n), dtype=c128, device=dev)
-        info_t = torch.empty(nf, dtype=torch.int32, device=dev)
-        h.sweep_dev(s.rom_id, freq_t, F_t, X_t, info_t, None, mode=mode, tol=tol,
-                    max_iter=max_iter, stream=side.cuda_stream)
-        infos.append(info_t)
-        # H = C X as (X^T C^T)^T: both operands of the batched product contiguous
-        CiT = torch.as_tensor(np.ascontiguousarray(s.C_iface.T), device=dev)
-        Hi = torch.matmul(X_t, CiT).transpose(1, 2)         # (nf, ni, ncol)
-        Hn = None
-        if s.C_internal.shape[0]:
-            CnT = torch.as_tensor(np.ascontiguousarray(s.C_internal.T), device=dev)
-            Hn = torch.matmul(X_t, CnT).transpose(1, 2)     # (nf, nn, ncol)
         u0 = torch.as_tensor(s.u0[:, None], device=dev)
         f = torch.as_tensor(s.f[:, None], device=dev)
+        if isinstance(s, MeasuredSubstructure):
+            # the same column layout as a simulated model: [H u0f | H_i* | H_*l]
+            Hii = torch.as_tensor(s.H_ii, device=dev)
+            Hil = torch.as_tensor(s.H_il, device=dev)
+            y0 = torch.matmul(Hii, u0) + (torch.matmul(Hil, f) if nl else 0)
+            Hi = torch.cat([y0, Hii, Hil], dim=2)
+            Hn = None
+            if s.H_ni.shape[1]:
+                Hn = torch.cat([torch.zeros((nf, s.H_ni.shape[1], 1), dtype=c128, device=dev),
+                                torch.as_tensor(s.H_ni, device=dev),
+                                torch.as_tensor(s.H_nl, device=dev)], dim=2)
+            infos.append(torch.zeros(nf, dtype=torch.int32, device=dev))
+        else:
+            Hi, Hn, info_t = _simulated_frfs(h, s, freq_t, nf, mode, tol, max_iter, dev, side)
+            infos.append(info_t)
         y0 = Hi[:, :, :1].contiguous()                      # H_ii u0 + H_il f
         dYs = dY_t[:, off:off + ni].contiguous()
         dUs = dU_t[:, off:off + ni].contiguous()
@@ -133,6 +161,36 @@ def _couple_on(h, subs, dRdY, dRdU, freq, mode, tol, max_iter, dev, side):
         J[:, :, off:off + ni] += torch.matmul(dYs, Hi[:, :, 1:1 + ni].contiguous())
         parts.append((Hi, Hn, u0, f, ni, nl, off))
         off += ni
+    return _correct(h, J, r, parts, infos, nf, Nu, mode, tol, max_iter, dev, side)
+
+
+def _simulated_frfs(h, s, freq_t, nf, mode, tol, max_iter, dev, side):
+    """(C_iface X, C_internal X or None, info) for X = A^{-1} [load | B_iface |
+    B_load] at every frequency: one sweep of the ROM, then device GEMMs."""
+    import torch
+
+    # right-hand sides (column-major n x ncol, stored as (ncol, n)): the
+    # load B_iface u0 + B_load f, then the columns of B_iface and B_load
+    rhs = np.hstack([(s.B_iface @ s.u0 + s.B_load @ s.f)[:, None], s.B_iface, s.B_load])
+    F_t = torch.as_tensor(np.ascontiguousarray(rhs.T), device=dev)
+    X_t = torch.empty((nf, rhs.shape[1], s.n), dtype=torch.complex128, device=dev)
+    info_t = torch.empty(nf, dtype=torch.int32, device=dev)
+    h.sweep_dev(s.rom_id, freq_t, F_t, X_t, info_t, None, mode=mode, tol=tol,
+                max_iter=max_iter, stream=side.cuda_stream)
+    # H = C X as (X^T C^T)^T: both operands of the batched product contiguous
+    CiT = torch.as_tensor(np.ascontiguousarray(s.C_iface.T), device=dev)
+    Hi = torch.matmul(X_t, CiT).transpose(1, 2)             # (nf, ni, ncol)
+    Hn = None
+    if s.C_internal.shape[0]:
+        CnT = torch.as_tensor(np.ascontiguousarray(s.C_internal.T), device=dev)
+        Hn = torch.matmul(X_t, CnT).transpose(1, 2)         # (nf, nn, ncol)
+    return Hi, Hn, info_t
+
+
+def _correct(h, J, r, parts, infos, nf, Nu, mode, tol, max_iter, dev, side):
+    import torch
+
+    c128 = torch.complex128
     # delta = J^{-1} r by the LU kernel, on the same stream as the GEMMs above
     Jc = J.transpose(1, 2).contiguous()
     rc = r.transpose(1, 2).contiguous()
@@ -160,4 +218,4 @@ def _couple_on(h, subs, dRdY, dRdU, freq, mode, tol, max_iter, dev, side):
             "info_sub": [i.cpu().numpy() for i in infos], "info_J": info_J.cpu().numpy()}
 
 
-__all__ = ["Substructure", "couple", "StcError"]
+__all__ = ["Substructure", "MeasuredSubstructure", "couple", "StcError"]

@@ -99,6 +99,25 @@ def test_oracle_coupling_equals_monolithic(sizes, ni):
     check_against_monolithic(subs, gmaps, Y, Yint, Xm, 1e-10)
 
 
+def measured_version(s, freq):
+    """The experimental-model form of a simulated substructure: its FRF
+    blocks at every frequency (as a test rig would measure them)."""
+    blocks = [ijcsa.frf_blocks(s, 2 * np.pi * fz) for fz in freq]
+    m = {key: np.stack([b[q] for b in blocks]) for q, key in enumerate(("H_ii", "H_il", "H_ni", "H_nl"))}
+    m.update(u0=s["u0"], f=s["f"])
+    return m
+
+
+def test_oracle_measured_member_equals_monolithic():
+    """A chain whose middle member enters only through its FRFs (an
+    experimental model, IJSA.cu:1011-1187) still gives the monolithic solution."""
+    subs, glob, gmaps, dRdY, dRdU, F = chain_case((9, 14, 11), 2, seed=9)
+    mixed = [subs[0], measured_version(subs[1], FREQ), subs[2]]
+    U, Y, Yint = ijcsa.couple(mixed, dRdY, dRdU, FREQ)
+    Xm = monolithic(glob, subs[0]["eta"], FREQ, F)
+    check_against_monolithic(subs, gmaps, Y, Yint, Xm, 1e-10)
+
+
 def _couple_without_dRdU_in_J(subs, dRdY, dRdU, freq):
     """ijcsa.couple's steps with the dRdU term dropped from the Jacobian
     (a small shift keeps J invertible); returns substructure 0's internal
@@ -272,3 +291,29 @@ def test_gpu_coupling_dense_io_matches_oracle(handle, mode):
     assert np.abs(out["u"] - U).max() <= gate_u * np.abs(U).max()
     for a, b in zip(out["Y_internal"], Yint):
         assert np.abs(a - b).max() <= gate * np.abs(b).max()
+
+
+@pytest.mark.gpu
+def test_gpu_coupling_with_measured_member(handle):
+    """Simulated ends, measured middle member: GPU step against the oracle and
+    the monolithic solve."""
+    from staccato_amd.coupling import MeasuredSubstructure, couple
+    subs, glob, gmaps, dRdY, dRdU, F = chain_case((20, 26, 18), 3, seed=31)
+    freq = np.linspace(2.0, 800.0, 23)
+    m = measured_version(subs[1], freq)
+    for k in (0, 2):
+        s = subs[k]
+        handle.rom_set(70 + k, s["K"], s["M"], s["D"], s["eta"])
+    gsubs = [Substructure.from_dofs(70, 20, subs[0]["iface"], subs[0]["load"], subs[0]["internal"],
+                                    subs[0]["u0"], subs[0]["f"]),
+             MeasuredSubstructure(m["H_ii"], m["H_il"], m["H_ni"], m["H_nl"], m["u0"], m["f"]),
+             Substructure.from_dofs(72, 18, subs[2]["iface"], subs[2]["load"], subs[2]["internal"],
+                                    subs[2]["u0"], subs[2]["f"])]
+    print("coupling with a measured member", flush=True)
+    out = couple(handle, gsubs, dRdY, dRdU, freq)
+    U, Y, Yint = ijcsa.couple([subs[0], m, subs[2]], dRdY, dRdU, freq)
+    assert (out["info_J"] == 0).all() and all((i == 0).all() for i in out["info_sub"])
+    assert np.abs(out["Y"] - Y).max() <= 1e-10 * np.abs(Y).max()
+    assert np.abs(out["u"] - U).max() <= 1e-9 * np.abs(U).max()
+    Xm = monolithic(glob, subs[0]["eta"], freq, F)
+    check_against_monolithic(subs, gmaps, out["Y"], out["Y_internal"], Xm, 1e-10)
