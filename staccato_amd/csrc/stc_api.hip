@@ -951,7 +951,7 @@ int stc_gesv_batched_dev(stc_handle h, int n, int batch, const stc_z64* d_A, lon
                          int nrhs, const stc_z64* d_B, int ldb, long long strideB, stc_z64* d_X,
                          int ldx, int* d_info, double* d_berr, int mode, double tol,
                          int max_iter, void* stream) {
-    if (!h || n < 1 || batch < 0 || nrhs < 1 || check_mode(mode)) return STC_EINVAL;
+    if (!h || n < 1 || n > 1024 || batch < 0 || nrhs < 1 || check_mode(mode)) return STC_EINVAL;
     if (strideA < (long long)n * n || ldb < n || ldx < n ||
         (strideB != 0 && strideB < (long long)ldb * nrhs))
         return STC_EINVAL;
@@ -993,7 +993,7 @@ int stc_gesv_batched(stc_handle h, int n, int batch, const stc_z64* A, int lda, 
                      int nrhs, const stc_z64* B, int ldb, long long strideB, stc_z64* X, int ldx,
                      long long strideX, int* info, double* berr, int mode, double tol,
                      int max_iter, stc_stats* stats) {
-    if (!h || n < 1 || batch < 0 || nrhs < 1 || check_mode(mode)) return STC_EINVAL;
+    if (!h || n < 1 || n > 1024 || batch < 0 || nrhs < 1 || check_mode(mode)) return STC_EINVAL;
     if (lda < n || ldb < n || ldx < n || strideA < (long long)lda * n ||
         (strideB != 0 && strideB < (long long)ldb * nrhs) || strideX < (long long)ldx * nrhs)
         return STC_EINVAL;

@@ -317,3 +317,29 @@ def test_gpu_coupling_with_measured_member(handle):
     assert np.abs(out["u"] - U).max() <= 1e-9 * np.abs(U).max()
     Xm = monolithic(glob, subs[0]["eta"], freq, F)
     check_against_monolithic(subs, gmaps, out["Y"], out["Y_internal"], Xm, 1e-10)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["plain", "refined"])
+def test_gesv_batched_largest_size_and_limits(handle, mode):
+    """n = 1024 (the largest system the kernel takes), an empty batch, and
+    n = 1025 rejected before any launch."""
+    from staccato_amd._lib import STC_EINVAL
+    rng = np.random.default_rng(8)
+    n, nb = 1024, 3
+    A = _batch(rng, nb, n)
+    B = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    print(f"gesv_batched n=1024 mode={mode}", flush=True)
+    X, info, be, _ = handle.gesv_batched(A, B, mode=mode, tol=1e-12 if mode == "refined" else 0.0)
+    assert (info == 0).all()
+    Xr = np.linalg.solve(A, np.broadcast_to(B, (nb, n, 2)))
+    assert np.abs(X - Xr).max() <= 1e-9 * np.abs(Xr).max()
+    assert be.max() <= 1e-12
+    lib = handle.lib
+    z = np.zeros(4, np.complex128)
+    info1 = np.zeros(1, np.int32)
+    assert lib.stc_gesv_batched(handle.h, 1025, 1, z.ctypes.data, 1025, 1025 * 1025, 1,
+                                z.ctypes.data, 1025, 1025, z.ctypes.data, 1025, 1025,
+                                info1.ctypes.data, None, 0, 0.0, 0, None) == STC_EINVAL
+    assert lib.stc_gesv_batched(handle.h, 4, 0, None, 4, 16, 1, None, 4, 4, None, 4, 4, None,
+                                None, 0, 0.0, 0, None) == 0
