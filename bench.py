@@ -37,7 +37,7 @@ PEAK_HBM_GBS = 8000.0
 # the same counters read for a 256 MiB device-to-device copy (FETCH_SIZE reads half
 # the bytes copied: x2.0; WRITE_SIZE reads them exactly: x1.0).  The summary and the
 # commands are in profiles/sweep_kernel_n256_pmc.txt; re-measure when the kernel changes.
-TRAFFIC_KIB = {"refined": (9.813e6, 14.23e6), "plain": (18.97e6, 34.96e6)}
+TRAFFIC_KIB = {"refined": (9.884e6, 14.23e6), "plain": (18.90e6, 34.96e6)}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
