@@ -1009,15 +1009,23 @@ int stc_gesv_batched(stc_handle h, int n, int batch, const stc_z64* A, int lda, 
     if ((rc = h->X.ensure(sizeof(stc_z64) * (size_t)n * nrhs * batch))) return rc;
     if ((rc = h->info.ensure(sizeof(int) * batch))) return rc;
     if (berr && (rc = h->berr.ensure(sizeof(double) * (size_t)batch * nrhs))) return rc;
-    // pack the matrices and right-hand sides (leading dimension n)
-    for (int j = 0; j < batch; ++j)
-        HIP_TRY(hipMemcpy2DAsync(h->A.as<stc_z64>() + (size_t)j * nn, sizeof(stc_z64) * n,
-                                 A + (size_t)j * strideA, sizeof(stc_z64) * lda,
-                                 sizeof(stc_z64) * n, n, hipMemcpyHostToDevice, st));
-    for (size_t j = 0; j < nb; ++j)
-        HIP_TRY(hipMemcpy2DAsync(h->F.as<stc_z64>() + j * n * nrhs, sizeof(stc_z64) * n,
-                                 B + j * strideB, sizeof(stc_z64) * ldb, sizeof(stc_z64) * n,
-                                 nrhs, hipMemcpyHostToDevice, st));
+    // pack the matrices and right-hand sides (leading dimension n); one copy
+    // when the caller's layout is packed already
+    if (lda == n && strideA == (long long)nn)
+        HIP_TRY(hipMemcpyAsync(h->A.p, A, sizeof(stc_z64) * nn * batch, hipMemcpyHostToDevice, st));
+    else
+        for (int j = 0; j < batch; ++j)
+            HIP_TRY(hipMemcpy2DAsync(h->A.as<stc_z64>() + (size_t)j * nn, sizeof(stc_z64) * n,
+                                     A + (size_t)j * strideA, sizeof(stc_z64) * lda,
+                                     sizeof(stc_z64) * n, n, hipMemcpyHostToDevice, st));
+    if (ldb == n && (nb == 1 || strideB == (long long)n * nrhs))
+        HIP_TRY(hipMemcpyAsync(h->F.p, B, sizeof(stc_z64) * n * nrhs * nb, hipMemcpyHostToDevice,
+                               st));
+    else
+        for (size_t j = 0; j < nb; ++j)
+            HIP_TRY(hipMemcpy2DAsync(h->F.as<stc_z64>() + j * n * nrhs, sizeof(stc_z64) * n,
+                                     B + j * strideB, sizeof(stc_z64) * ldb, sizeof(stc_z64) * n,
+                                     nrhs, hipMemcpyHostToDevice, st));
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1033,10 +1041,13 @@ int stc_gesv_batched(stc_handle h, int n, int batch, const stc_z64* A, int lda, 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    for (int j = 0; j < batch; ++j)
-        HIP_TRY(hipMemcpy2D(X + (size_t)j * strideX, sizeof(stc_z64) * ldx,
-                            h->X.as<stc_z64>() + (size_t)j * n * nrhs, sizeof(stc_z64) * n,
-                            sizeof(stc_z64) * n, nrhs, hipMemcpyDeviceToHost));
+    if (ldx == n && strideX == (long long)n * nrhs)
+        HIP_TRY(hipMemcpy(X, h->X.p, sizeof(stc_z64) * n * nrhs * batch, hipMemcpyDeviceToHost));
+    else
+        for (int j = 0; j < batch; ++j)
+            HIP_TRY(hipMemcpy2D(X + (size_t)j * strideX, sizeof(stc_z64) * ldx,
+                                h->X.as<stc_z64>() + (size_t)j * n * nrhs, sizeof(stc_z64) * n,
+                                sizeof(stc_z64) * n, nrhs, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(info, h->info.p, sizeof(int) * batch, hipMemcpyDeviceToHost));
     if (berr)
         HIP_TRY(hipMemcpy(berr, h->berr.p, sizeof(double) * batch * nrhs, hipMemcpyDeviceToHost));

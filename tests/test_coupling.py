@@ -343,3 +343,38 @@ def test_gesv_batched_largest_size_and_limits(handle, mode):
                                 info1.ctypes.data, None, 0, 0.0, 0, None) == STC_EINVAL
     assert lib.stc_gesv_batched(handle.h, 4, 0, None, 4, 16, 1, None, 4, 4, None, 4, 4, None,
                                 None, 0, 0.0, 0, None) == 0
+
+
+@pytest.mark.gpu
+def test_gesv_batched_padded_layouts(handle):
+    """Leading dimensions and batch strides larger than the packed ones (the
+    caller's storage of IJSA.cu-style pointer batches), through the C ABI."""
+    import ctypes
+    rng = np.random.default_rng(12)
+    n, nb, nrhs = 37, 5, 3
+    lda, sA = n + 3, (n + 3) * n + 5
+    ldb, sB = n + 2, (n + 2) * nrhs + 1
+    ldx, sX = n + 1, (n + 1) * nrhs + 2
+    A = _batch(rng, nb, n)
+    B = rng.standard_normal((nb, n, nrhs)) + 1j * rng.standard_normal((nb, n, nrhs))
+    Abuf = np.full(sA * nb, np.nan, np.complex128)
+    Bbuf = np.full(sB * nb, np.nan, np.complex128)
+    Xbuf = np.full(sX * nb, 7.0 + 7j, np.complex128)
+    for j in range(nb):
+        for c in range(n):
+            Abuf[j * sA + c * lda: j * sA + c * lda + n] = A[j][:, c]
+        for c in range(nrhs):
+            Bbuf[j * sB + c * ldb: j * sB + c * ldb + n] = B[j][:, c]
+    info = np.zeros(nb, np.int32)
+    print("gesv_batched padded layouts", flush=True)
+    rc = handle.lib.stc_gesv_batched(handle.h, n, nb, Abuf.ctypes.data, lda, sA, nrhs,
+                                     Bbuf.ctypes.data, ldb, sB, Xbuf.ctypes.data, ldx, sX,
+                                     info.ctypes.data, None, 0, 0.0, 0, None)
+    assert rc == 0 and (info == 0).all()
+    Xr = np.linalg.solve(A, B)
+    for j in range(nb):
+        for c in range(nrhs):
+            col = Xbuf[j * sX + c * ldx: j * sX + c * ldx + n]
+            assert np.abs(col - Xr[j][:, c]).max() <= 1e-11 * np.abs(Xr).max()
+            # padding between columns and blocks untouched
+            assert (Xbuf[j * sX + c * ldx + n: j * sX + (c + 1) * ldx] == 7.0 + 7j).all()
