@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc", "stc_api.hip")
 OUT_DIR = os.path.join(HERE, "lib")
 OUT = os.path.join(OUT_DIR, "libstaccato_sweep.so")
-HOST_SRC = os.path.join(HERE, "csrc", "host", "KrylovROMSweep.cpp")
+HOST_SRC = [os.path.join(HERE, "csrc", "host", f) for f in ("KrylovROMSweep.cpp", "IJCSACoupling.cpp")]
 HOST_OUT = os.path.join(OUT_DIR, "libstaccato_host.so")
 
 HIPCC_FLAGS = [
@@ -38,7 +38,7 @@ def build_library(verbose=False):
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, OUT)
     # C++ mirror of the reference's sweep entry, over the C ABI
-    hcmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", HOST_OUT + ".tmp", HOST_SRC,
+    hcmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", HOST_OUT + ".tmp", *HOST_SRC,
             "-L" + OUT_DIR, "-lstaccato_sweep", "-Wl,-rpath,$ORIGIN"]
     r = subprocess.run(hcmd, capture_output=True, text=True)
     if r.returncode != 0:

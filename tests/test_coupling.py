@@ -378,3 +378,62 @@ def test_gesv_batched_padded_layouts(handle):
             assert np.abs(col - Xr[j][:, c]).max() <= 1e-11 * np.abs(Xr).max()
             # padding between columns and blocks untouched
             assert (Xbuf[j * sX + c * ldx + n: j * sX + (c + 1) * ldx] == 7.0 + 7j).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["plain", "refined"])
+def test_cpp_host_coupling_matches_oracle(mode):
+    """The C++ host mirror (IJCSACoupling, through stc_host_ijcsa_couple)
+    against the oracle and the monolithic solve."""
+    import ctypes
+    from staccato_amd import _lib
+    _lib.load()
+    host = ctypes.CDLL(_lib.LIB_PATH.replace("libstaccato_sweep.so", "libstaccato_host.so"))
+    subs, glob, gmaps, dRdY, dRdU, F = chain_case((20, 26, 18), 3, seed=41)
+    freq = np.linspace(3.0, 850.0, 31)
+    dims, Ks, Ms, Ds, Bs, Cs, us, fs, etas = [], [], [], [], [], [], [], [], []
+    for s in subs:
+        n = s["K"].shape[0]
+        Bi, Bl, Ci, Cn = ijcsa.io_matrices(s)
+        dims += [n, Bi.shape[1], Bl.shape[1], Cn.shape[0]]
+        Ks.append(s["K"].astype(np.complex128).ravel("F"))
+        Ms.append(s["M"].astype(np.complex128).ravel("F"))
+        Ds.append(s["D"].astype(np.complex128).ravel("F"))
+        Bs.append(np.hstack([Bi, Bl]).ravel("F"))
+        Cs.append(np.vstack([Ci, Cn]).ravel("F"))
+        us.append(s["u0"])
+        fs.append(np.asarray(s["f"], np.complex128))
+        etas.append(s["eta"])
+    cat = lambda xs: np.ascontiguousarray(np.concatenate(xs), dtype=np.complex128)
+    dims = np.array(dims, np.int32)
+    K, M, D, B, C, u0, f = map(cat, (Ks, Ms, Ds, Bs, Cs, us, fs))
+    eta = np.array(etas, np.float64)
+    Nu = dRdY.shape[0]
+    nint = sum(dims[3::4])
+    U = np.zeros((freq.size, Nu), np.complex128)
+    Y = np.zeros((freq.size, Nu), np.complex128)
+    Yint = np.zeros(freq.size * nint, np.complex128)
+    infoJ = np.zeros(freq.size, np.int32)
+    dY = np.asfortranarray(dRdY, dtype=np.complex128)
+    dU = np.asfortranarray(dRdU, dtype=np.complex128)
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)
+    fn = host.stc_host_ijcsa_couple
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 11 + \
+        [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double] + [ctypes.c_void_p] * 4
+    print(f"C++ coupling mode={mode}", flush=True)
+    rc = fn(0, len(subs), p(dims), p(K), p(M), p(D), p(eta), p(B), p(C), p(u0), p(f), p(dY), p(dU),
+            freq.size, p(freq), 1 if mode == "refined" else 0, 1e-12 if mode == "refined" else 0.0,
+            p(U), p(Y), p(Yint), p(infoJ))
+    assert rc == 0 and (infoJ == 0).all()
+    Uo, Yo, Yinto = ijcsa.couple(subs, dRdY, dRdU, freq)
+    gate = 1e-10 if mode == "plain" else 1e-8
+    assert np.abs(Y - Yo).max() <= gate * np.abs(Yo).max()
+    assert np.abs(U - Uo).max() <= (1e-9 if mode == "plain" else 1e-6) * np.abs(Uo).max()
+    o, Yint_l = 0, []
+    for k, s in enumerate(subs):
+        nn = dims[4 * k + 3]
+        Yint_l.append(Yint[o:o + freq.size * nn].reshape(freq.size, nn))
+        o += freq.size * nn
+    Xm = monolithic(glob, subs[0]["eta"], freq, F)
+    check_against_monolithic(subs, gmaps, Y, Yint_l, Xm, gate)
