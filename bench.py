@@ -278,10 +278,18 @@ def other_workload(name, steps, warmup):
                 res = couple(h, subs, dRdY, dRdU, freq, mode=mode, tol=tol)
             dt = (time.perf_counter() - t0) / steps
             bad = int((res["info_J"] != 0).sum()) + sum(int((i != 0).sum()) for i in res["info_sub"])
+            # the interface residual after the Newton step (compatibility rows
+            # relative to the responses, equilibrium rows to the forces)
+            R = res["Y"] @ dRdY.T + res["u"] @ dRdU.T
+            comp = max(float(np.abs(R[:, row0:row0 + nj]).max()) for row0 in (0, 2 * nj))
+            equi = max(float(np.abs(R[:, row0:row0 + nj]).max()) for row0 in (nj, 3 * nj))
+            comp /= float(np.abs(res["Y"]).max())
+            equi /= float(np.abs(res["u"]).max())
             run = lambda: h.gesv_batched_dev(Js, rs, ds, inf, mode=mode, tol=tol,
                                              stream=stream.cuda_stream)
             wall, ev = time_gpu(run, steps, warmup, stream)
             out[mode] = {"coupled_pts_per_s": nfq / dt, "points_failed": bad,
+                         "compatibility_residual_rel": comp, "equilibrium_residual_rel": equi,
                          "interface_solve_pts_per_s": nfq * steps / wall,
                          "interface_solve_kernel_ms": 1e3 * ev / steps}
         out["config"] = {"substructures": 3, "n": n, "shared_dofs_per_joint": nj, "Nu": int(Nu),

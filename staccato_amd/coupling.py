@@ -5,13 +5,13 @@ of the interface residual per frequency, with every array device resident:
 
 * FRFs of every substructure at its interface, load and internal-output DoFs,
   H_s(w) = C_s A_s(w)^{-1} B_s (IJSA.cu:830-917): one sweep of the ROM
-  (Handle.sweep_dev) with right-hand sides [B_iface u0 + B_load f | B_iface |
-  B_load], then C_iface X and C_internal X as batched device GEMMs;
+  (Handle.sweep_dev) with right-hand sides [B_iface | B_load], then C_iface X
+  and C_internal X as batched device GEMMs;
 * initial interface solution Y_s = H_ii u_s + H_il f_s (IJSA.cu:920-946),
   residual r = sum_s dRdY_s Y_s + dRdU_s u_s (IJSA.cu:948-968) and interface
   Jacobian J = dRdU + sum_s dRdY_s H_ii (IJSA.cu:523-527, 970-981): batched
   device GEMMs;
-* correction delta = J^{-1} r with the pivoted LU kernel
+* correction delta = J^{-1} r with the pivoted LU kernel in FP64
   (Handle.gesv_batched_dev; IJSA.cu:1199-1208 factors J unpivoted);
 * update u_s -= delta_s (IJSA.cu:1349-1357) and the responses
   Y_s = H_ii u_s + H_il f_s, Y_int,s = H_ni u_s + H_nl f_s (IJSA.cu:1359-1411).
@@ -92,7 +92,9 @@ class MeasuredSubstructure:
 
 
 def couple(h, subs, dRdY, dRdU, freq, mode="plain", tol=0.0, max_iter=0):
-    """Coupled responses at the frequencies freq (Hz).  Returns a dict with
+    """Coupled responses at the frequencies freq (Hz); mode, tol and max_iter
+    apply to the substructure sweeps (the Jacobians are solved in FP64).
+    Returns a dict with
     u (nfreq, Nu) interface forces, Y (nfreq, Nu) interface responses,
     Y_internal [per substructure (nfreq, nn_s)], info_sub [per substructure
     (nfreq,)] and info_J (nfreq,) (0 = ok, k > 0 = zero pivot k, -1 = not
@@ -139,39 +141,37 @@ def _couple_on(h, subs, dRdY, dRdU, freq, mode, tol, max_iter, dev, side):
         ni, nl = s.u0.size, s.f.size
         u0 = torch.as_tensor(s.u0[:, None], device=dev)
         f = torch.as_tensor(s.f[:, None], device=dev)
+        # Hi = [H_ii | H_il] (nf, ni, ni+nl), Hn = [H_ni | H_nl] (nf, nn, ni+nl) or None
         if isinstance(s, MeasuredSubstructure):
-            # the same column layout as a simulated model: [H u0f | H_i* | H_*l]
-            Hii = torch.as_tensor(s.H_ii, device=dev)
-            Hil = torch.as_tensor(s.H_il, device=dev)
-            y0 = torch.matmul(Hii, u0) + (torch.matmul(Hil, f) if nl else 0)
-            Hi = torch.cat([y0, Hii, Hil], dim=2)
+            Hi = torch.cat([torch.as_tensor(s.H_ii, device=dev),
+                            torch.as_tensor(s.H_il, device=dev)], dim=2)
             Hn = None
             if s.H_ni.shape[1]:
-                Hn = torch.cat([torch.zeros((nf, s.H_ni.shape[1], 1), dtype=c128, device=dev),
-                                torch.as_tensor(s.H_ni, device=dev),
+                Hn = torch.cat([torch.as_tensor(s.H_ni, device=dev),
                                 torch.as_tensor(s.H_nl, device=dev)], dim=2)
             infos.append(torch.zeros(nf, dtype=torch.int32, device=dev))
         else:
             Hi, Hn, info_t = _simulated_frfs(h, s, freq_t, nf, mode, tol, max_iter, dev, side)
             infos.append(info_t)
-        y0 = Hi[:, :, :1].contiguous()                      # H_ii u0 + H_il f
+        uf0 = torch.cat([u0, f], dim=0)
+        y0 = torch.matmul(Hi, uf0)                          # H_ii u0 + H_il f (IJSA.cu:920-946)
         dYs = dY_t[:, off:off + ni].contiguous()
         dUs = dU_t[:, off:off + ni].contiguous()
         r += torch.matmul(dYs, y0) + torch.matmul(dUs, u0)
-        J[:, :, off:off + ni] += torch.matmul(dYs, Hi[:, :, 1:1 + ni].contiguous())
+        J[:, :, off:off + ni] += torch.matmul(dYs, Hi[:, :, :ni].contiguous())
         parts.append((Hi, Hn, u0, f, ni, nl, off))
         off += ni
     return _correct(h, J, r, parts, infos, nf, Nu, mode, tol, max_iter, dev, side)
 
 
 def _simulated_frfs(h, s, freq_t, nf, mode, tol, max_iter, dev, side):
-    """(C_iface X, C_internal X or None, info) for X = A^{-1} [load | B_iface |
-    B_load] at every frequency: one sweep of the ROM, then device GEMMs."""
+    """(C_iface X, C_internal X or None, info) for X = A^{-1} [B_iface | B_load]
+    at every frequency: one sweep of the ROM, then device GEMMs."""
     import torch
 
-    # right-hand sides (column-major n x ncol, stored as (ncol, n)): the
-    # load B_iface u0 + B_load f, then the columns of B_iface and B_load
-    rhs = np.hstack([(s.B_iface @ s.u0 + s.B_load @ s.f)[:, None], s.B_iface, s.B_load])
+    # right-hand sides: the columns of B_iface and B_load (column-major n x ncol,
+    # stored as (ncol, n))
+    rhs = np.hstack([s.B_iface, s.B_load])
     F_t = torch.as_tensor(np.ascontiguousarray(rhs.T), device=dev)
     X_t = torch.empty((nf, rhs.shape[1], s.n), dtype=torch.complex128, device=dev)
     info_t = torch.empty(nf, dtype=torch.int32, device=dev)
@@ -179,11 +179,11 @@ def _simulated_frfs(h, s, freq_t, nf, mode, tol, max_iter, dev, side):
                 max_iter=max_iter, stream=side.cuda_stream)
     # H = C X as (X^T C^T)^T: both operands of the batched product contiguous
     CiT = torch.as_tensor(np.ascontiguousarray(s.C_iface.T), device=dev)
-    Hi = torch.matmul(X_t, CiT).transpose(1, 2)             # (nf, ni, ncol)
+    Hi = torch.matmul(X_t, CiT).transpose(1, 2).contiguous()    # (nf, ni, ncol)
     Hn = None
     if s.C_internal.shape[0]:
         CnT = torch.as_tensor(np.ascontiguousarray(s.C_internal.T), device=dev)
-        Hn = torch.matmul(X_t, CnT).transpose(1, 2)         # (nf, nn, ncol)
+        Hn = torch.matmul(X_t, CnT).transpose(1, 2).contiguous()    # (nf, nn, ncol)
     return Hi, Hn, info_t
 
 
@@ -196,8 +196,10 @@ def _correct(h, J, r, parts, infos, nf, Nu, mode, tol, max_iter, dev, side):
     rc = r.transpose(1, 2).contiguous()
     delta = torch.empty_like(rc)
     info_J = torch.empty(nf, dtype=torch.int32, device=dev)
-    h.gesv_batched_dev(Jc, rc, delta, info_J, mode=mode, tol=tol, max_iter=max_iter,
-                       stream=side.cuda_stream)
+    # the interface Jacobians are solved in FP64 in either mode, as IJSA.cu does
+    # (cuDoubleComplex): their rows mix compliances (~1/k) with unit equilibrium
+    # rows, so a normwise complex64 tolerance would not hold compatibility
+    h.gesv_batched_dev(Jc, rc, delta, info_J, mode="plain", stream=side.cuda_stream)
     delta = delta.transpose(1, 2)                           # (nf, Nu, 1)
     U = torch.empty((nf, Nu, 1), dtype=c128, device=dev)
     Y = torch.empty((nf, Nu, 1), dtype=c128, device=dev)
@@ -207,9 +209,9 @@ def _correct(h, J, r, parts, infos, nf, Nu, mode, tol, max_iter, dev, side):
         U[:, off:off + ni] = u
         # [H_ii | H_il] [u; f] and [H_ni | H_nl] [u; f]
         uf = torch.cat([u, f.expand(nf, nl, 1)], dim=1)
-        Y[:, off:off + ni] = torch.matmul(Hi[:, :, 1:].contiguous(), uf)
+        Y[:, off:off + ni] = torch.matmul(Hi.contiguous(), uf)
         if Hn is not None:
-            Yint.append(torch.matmul(Hn[:, :, 1:].contiguous(), uf)[:, :, 0])
+            Yint.append(torch.matmul(Hn.contiguous(), uf)[:, :, 0])
         else:
             Yint.append(torch.zeros((nf, 0), dtype=c128, device=dev))
     side.synchronize()

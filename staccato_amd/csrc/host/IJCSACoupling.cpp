@@ -52,26 +52,32 @@ int IJCSACoupling::solve(const std::vector<double>& freq) {
         int rc = stc_rom_set_io(myHandle, s.romId, nin, reinterpret_cast<const stc_z64*>(s.B.data()),
                                 s.n, nout, reinterpret_cast<const stc_z64*>(s.C.data()), nout);
         if (rc) return rc;
-        // load [u; f]: its response y0 = H_ii u + H_il f comes with H itself
+        // H_s from the sweep (the load column's own response is not used: y0 is
+        // formed from H below, as IJSA.cu:920-946 does, so that r and J agree)
         std::vector<stc_z64> load(nin);
         for (int p = 0; p < ni; ++p) load[p] = {s.rhsInterface[p].real, s.rhsInterface[p].imag};
         for (int p = 0; p < nl; ++p) load[ni + p] = {s.rhsLoad[p].real, s.rhsLoad[p].imag};
-        std::vector<stc_z64> y0((size_t)nf * nout);
+        std::vector<stc_z64> ysweep((size_t)nf * nout);
         H[k].resize((size_t)nf * nout * nin);
         std::vector<int> info(nf);
-        rc = stc_back_transform(myHandle, s.romId, nf, freq.data(), load.data(), 1, y0.data(),
+        rc = stc_back_transform(myHandle, s.romId, nf, freq.data(), load.data(), 1, ysweep.data(),
                                 H[k].data(), info.data(), myMode, myTol, myMaxIter, nullptr);
         if (rc < 0) return rc;
         if (rc > 0 && status == STC_OK) status = rc;
         // r += dRdY_s y0_s + dRdU_s u_s;  J(:, block s) += dRdY_s H_ii
+        std::vector<cd> yf(ni);
         for (int f = 0; f < nf; ++f) {
             const stc_z64* Hf = &H[k][(size_t)f * nout * nin];
-            const stc_z64* yf = &y0[(size_t)f * nout];
             stc_z64* Jf = &J[f * NN];
+            for (int p = 0; p < ni; ++p) {  // y0 = H_ii u + H_il f
+                cd y = 0.0;
+                for (int q = 0; q < nin; ++q) y += zc(Hf[p + (size_t)q * nout]) * zc(load[q]);
+                yf[p] = y;
+            }
             for (int R = 0; R < Nu; ++R) {
                 cd acc = 0.0;
                 for (int p = 0; p < ni; ++p)
-                    acc += dY(R, off + p) * zc(yf[p]) + dU(R, off + p) * zc(s.rhsInterface[p]);
+                    acc += dY(R, off + p) * yf[p] + dU(R, off + p) * zc(s.rhsInterface[p]);
                 stc_z64& rr = r[(size_t)f * Nu + R];
                 rr = {rr.re + acc.real(), rr.im + acc.imag()};
                 for (int q = 0; q < ni; ++q) {
@@ -85,11 +91,13 @@ int IJCSACoupling::solve(const std::vector<double>& freq) {
         off += ni;
     }
 
-    // delta = J^{-1} r for every frequency on the GPU
+    // delta = J^{-1} r for every frequency on the GPU, in FP64 whatever the
+    // substructures' mode (IJSA.cu solves J in cuDoubleComplex): compliance rows
+    // beside unit equilibrium rows defeat a normwise complex64 tolerance
     std::vector<stc_z64> delta((size_t)Nu * nf);
     int rc = stc_gesv_batched(myHandle, Nu, nf, J.data(), Nu, (long long)NN, 1, r.data(), Nu, Nu,
-                              delta.data(), Nu, Nu, myInfoJ.data(), nullptr, myMode, myTol,
-                              myMaxIter, nullptr);
+                              delta.data(), Nu, Nu, myInfoJ.data(), nullptr, STC_PLAIN_FP64, 0.0,
+                              0, nullptr);
     if (rc < 0) return rc;
     if (rc > 0 && status == STC_OK) status = rc;
 

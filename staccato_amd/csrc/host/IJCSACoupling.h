@@ -9,7 +9,7 @@
 //   Y_s = H_ii u_s + H_il f_s (IJSA.cu:920-946);
 //   r = sum_s dRdY_s Y_s + dRdU_s u_s (IJSA.cu:948-968);
 //   J = dRdU + sum_s dRdY_s H_ii (IJSA.cu:523-527, 970-981);
-//   delta = J^{-1} r (IJSA.cu:1199-1208; here stc_gesv_batched, pivoted);
+//   delta = J^{-1} r (IJSA.cu:1199-1208; here stc_gesv_batched, pivoted, FP64);
 //   u_s -= delta_s (IJSA.cu:1349-1357), then Y_s and the internal responses
 //   H_ni u_s + H_nl f_s (IJSA.cu:1359-1411).
 // The substructure solves and the Jacobian solves run on the GPU; the
@@ -33,6 +33,7 @@ struct IJCSASubsystem {
 
 class IJCSACoupling {
 public:
+    // mode, tol, maxIter: the substructure sweeps (the Jacobians are solved in FP64)
     explicit IJCSACoupling(stc_handle handle, int mode = STC_PLAIN_FP64, double tol = 0.0,
                            int maxIter = 0);
 

@@ -437,3 +437,32 @@ def test_cpp_host_coupling_matches_oracle(mode):
         o += freq.size * nn
     Xm = monolithic(glob, subs[0]["eta"], freq, F)
     check_against_monolithic(subs, gmaps, Y, Yint_l, Xm, gate)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["plain", "refined"])
+def test_gpu_coupling_full_size_residual_vanishes(handle, mode):
+    """Three n=256 substructures (the sweep's size), 16 shared DoFs per joint:
+    after the Newton step the interface residual R(u) = dRdY Y + dRdU u is zero
+    (compatibility and equilibrium hold) at every frequency."""
+    from staccato_amd.coupling import couple
+    subs, glob, gmaps, dRdY, dRdU, F = chain_case((256, 256, 256), 16, seed=51, nl=4)
+    freq = np.linspace(1.0, 2000.0, 512)
+    gsubs = []
+    for k, s in enumerate(subs):
+        handle.rom_set(80 + k, s["K"], s["M"], s["D"], s["eta"])
+        gsubs.append(Substructure.from_dofs(80 + k, 256, s["iface"], s["load"], s["internal"][-8:],
+                                            s["u0"], s["f"]))
+    print(f"coupling 3 x n=256 mode={mode}", flush=True)
+    out = couple(handle, gsubs, dRdY, dRdU, freq, mode=mode, tol=1e-12 if mode == "refined" else 0)
+    assert (out["info_J"] == 0).all() and all((i == 0).all() for i in out["info_sub"])
+    Y, U = out["Y"], out["u"]
+    R = Y @ dRdY.T + U @ dRdU.T                     # (nf, Nu)
+    Nu, nj = dRdY.shape[0], 16
+    compat = np.concatenate([R[:, 0:nj], R[:, 2 * nj:3 * nj]], axis=1)
+    equil = np.concatenate([R[:, nj:2 * nj], R[:, 3 * nj:4 * nj]], axis=1)
+    # R(u) is formed from the same FRFs as J, and J is solved in FP64 in both
+    # modes: the residual vanishes to FP64 accuracy whatever the sweeps' mode
+    gate_c, gate_e = 1e-10, 1e-12
+    assert np.abs(compat).max() <= gate_c * np.abs(Y).max()
+    assert np.abs(equil).max() <= gate_e * np.abs(U).max()
