@@ -97,8 +97,8 @@ def couple(h, subs, dRdY, dRdU, freq, mode="plain", tol=0.0, max_iter=0):
     Returns a dict with
     u (nfreq, Nu) interface forces, Y (nfreq, Nu) interface responses,
     Y_internal [per substructure (nfreq, nn_s)], info_sub [per substructure
-    (nfreq,)] and info_J (nfreq,) (0 = ok, k > 0 = zero pivot k, -1 = not
-    refined to tol)."""
+    (nfreq,): 0 = ok, k > 0 = zero pivot k, -1 = not refined to tol] and
+    info_J (nfreq,) (0 = ok, k > 0 = zero pivot k)."""
     import torch
 
     if not torch.cuda.is_available():
@@ -161,7 +161,7 @@ def _couple_on(h, subs, dRdY, dRdU, freq, mode, tol, max_iter, dev, side):
         J[:, :, off:off + ni] += torch.matmul(dYs, Hi[:, :, :ni].contiguous())
         parts.append((Hi, Hn, u0, f, ni, nl, off))
         off += ni
-    return _correct(h, J, r, parts, infos, nf, Nu, mode, tol, max_iter, dev, side)
+    return _correct(h, J, r, parts, infos, nf, Nu, dev, side)
 
 
 def _simulated_frfs(h, s, freq_t, nf, mode, tol, max_iter, dev, side):
@@ -187,7 +187,7 @@ def _simulated_frfs(h, s, freq_t, nf, mode, tol, max_iter, dev, side):
     return Hi, Hn, info_t
 
 
-def _correct(h, J, r, parts, infos, nf, Nu, mode, tol, max_iter, dev, side):
+def _correct(h, J, r, parts, infos, nf, Nu, dev, side):
     import torch
 
     c128 = torch.complex128
@@ -209,9 +209,9 @@ def _correct(h, J, r, parts, infos, nf, Nu, mode, tol, max_iter, dev, side):
         U[:, off:off + ni] = u
         # [H_ii | H_il] [u; f] and [H_ni | H_nl] [u; f]
         uf = torch.cat([u, f.expand(nf, nl, 1)], dim=1)
-        Y[:, off:off + ni] = torch.matmul(Hi.contiguous(), uf)
+        Y[:, off:off + ni] = torch.matmul(Hi, uf)
         if Hn is not None:
-            Yint.append(torch.matmul(Hn.contiguous(), uf)[:, :, 0])
+            Yint.append(torch.matmul(Hn, uf)[:, :, 0])
         else:
             Yint.append(torch.zeros((nf, 0), dtype=c128, device=dev))
     side.synchronize()
