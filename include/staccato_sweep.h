@@ -9,7 +9,10 @@
  *
  * from the ROM operators K, M, D and solves A(omega) X = F by LU with partial
  * pivoting, entirely on one AMD Instinct MI355X (gfx950).  A(omega) is
- * assembled inside the factorisation kernels and never stored per frequency.
+ * assembled inside the factorisation kernel: small systems are factored in
+ * LDS; larger ones in a per-workgroup factor slot of device memory that is
+ * reused point after point (its traffic reaches HBM once the resident slots
+ * exceed the caches), so no buffer per frequency is ever allocated.
  *
  * What this replaces in the reference (DrStS/STACCATO):
  *   - the body of the frequency loop of KrylovROMSubstructure::backTransformKMOR
@@ -31,7 +34,9 @@
  *   - pivots are 1-based 32-bit ints with LAPACK zgetrf meaning
  *     (LP64 lapack_int, cmake/modules/FindMKL.cmake:57-72);
  *   - per-frequency info: 0 = solved, k > 0 = U(k,k) is exactly zero
- *     (LAPACK zgetrf info), -1 = iterative refinement did not reach tol.
+ *     (LAPACK zgetrf info), -1 = iterative refinement did not reach tol,
+ *     -2 = the point could not be run (internal; X is NaN and the call
+ *     returns STC_EDEVICE).
  *     The reference ignores zgetrf's info (KrylovROMSubstructure.cpp:1428);
  *     this library reports it and never calls exit().
  *
@@ -39,7 +44,11 @@
  * Use one handle per host thread; calls on one handle are not thread safe.
  * Calls taking host pointers are synchronous.  The *_dev calls take device
  * pointers and a stream (a hipStream_t passed as void*, NULL = the handle's
- * stream) and are asynchronous with respect to the host.
+ * stream) and are asynchronous with respect to the host.  Launches of one
+ * handle share its factor slots, so each waits (on the device) for the
+ * handle's previous launch when it goes to a different stream.  A *_dev call
+ * that needs larger factor slots than any earlier call on the handle waits on
+ * the host for the previous launch and reallocates them.
  */
 #ifndef STACCATO_SWEEP_H
 #define STACCATO_SWEEP_H
@@ -63,6 +72,7 @@ enum {
     STC_ENOMEM = -2,   /* device allocation failed                        */
     STC_EDEVICE = -3,  /* HIP runtime error                               */
     STC_ENOROM = -4,   /* rom_id not set on this handle                   */
+    STC_EBUSY = -5,    /* rom or handle still used by a live plan         */
     STC_ESINGULAR = 1, /* at least one frequency had info > 0             */
     STC_ENOCONV = 2    /* at least one frequency had info == -1           */
 };
@@ -86,6 +96,7 @@ typedef struct {
 
 /* Create a handle on HIP device `device` (ordinal). */
 int stc_create(stc_handle* h, int device);
+/* STC_EBUSY while plans created on the handle are alive. */
 int stc_destroy(stc_handle h);
 const char* stc_strerror(int code);
 /* 1 when the library was built for and runs on gfx950, else 0 (no GPU). */
@@ -98,7 +109,9 @@ int stc_device_ok(stc_handle h);
  * (KrylovROMSubstructure.cpp:332-333, :996-997).  Replaces the members
  * myKComplexReduced / myMComplexReduced / myDComplexReduced
  * (src/analysis/KrylovROMSubstructure.h:212-216).  rom_id >= 0; setting an id
- * again replaces it. */
+ * again replaces it (and drops its B_R / C_R when n changes).  Non-finite
+ * entries are rejected (STC_EINVAL); a ROM read by a live plan cannot be
+ * replaced or cleared (STC_EBUSY). */
 int stc_rom_set(stc_handle h, int rom_id, int n, const stc_z64* K,
                 const stc_z64* M, const stc_z64* D, int ld, double eta);
 
@@ -166,8 +179,11 @@ int stc_sweep_batch(stc_handle h, int nsys, const stc_sys_desc* sys, int mode,
                     double tol, int max_iter, stc_stats* stats);
 
 /* A plan holds a device-side work queue for a batch whose stc_sys_desc arrays
- * are DEVICE pointers, ordered longest-first, and a hipGraph of the launch.
- * stc_plan_execute replays it on `stream` (NULL = handle stream). */
+ * are DEVICE pointers, ordered longest-first, its own factor slots and queue
+ * head, and a hipGraph of the launch.  stc_plan_execute replays it on `stream`
+ * (NULL = handle stream); replays of one plan are ordered with each other.
+ * The plan keeps its ROMs: stc_rom_set / stc_rom_clear on them, and
+ * stc_destroy of the handle, return STC_EBUSY until stc_plan_destroy. */
 int stc_plan_create(stc_handle h, int nsys, const stc_sys_desc* dev_sys,
                     int mode, double tol, int max_iter, stc_plan* plan);
 int stc_plan_execute(stc_plan plan, void* stream);

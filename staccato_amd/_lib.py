@@ -15,6 +15,7 @@ STC_EINVAL = -1
 STC_ENOMEM = -2
 STC_EDEVICE = -3
 STC_ENOROM = -4
+STC_EBUSY = -5
 STC_ESINGULAR = 1
 STC_ENOCONV = 2
 

@@ -1,21 +1,29 @@
 """Build libstaccato_sweep.so for gfx950 with hipcc (in-tree, no JIT cache).
 
+Every sweep-kernel instantiation (factor type x rows per lane) is compiled in
+its own hipcc process, in parallel, then linked with the host API.
+
     python -m staccato_amd.build
 """
 import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "stc_api.hip")
+CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
+OBJ_DIR = os.path.join(OUT_DIR, "obj")
 OUT = os.path.join(OUT_DIR, "libstaccato_sweep.so")
-HOST_SRC = [os.path.join(HERE, "csrc", "host", f) for f in ("KrylovROMSweep.cpp", "IJCSACoupling.cpp")]
+HOST_SRC = [os.path.join(CSRC, "host", f) for f in ("KrylovROMSweep.cpp", "IJCSACoupling.cpp")]
 HOST_OUT = os.path.join(OUT_DIR, "libstaccato_host.so")
 
+# (refine, rows per lane) pairs of sweep_kernel, one object each
+INSTANCES = [(r, m) for r in (0, 1) for m in (1, 2, 4, 8, 16)]
+
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions stay off: the compiler would otherwise pair
     # float arithmetic into v_pk_* forms
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -29,24 +37,44 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build_library(verbose=False):
-    os.makedirs(OUT_DIR, exist_ok=True)
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, SRC]
+def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+        raise RuntimeError("compile failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return cmd
+
+
+def build_library(verbose=False, jobs=None):
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    cc = hipcc()
+    jobs = jobs or min(16, max(1, os.cpu_count() or 1))
+    cmds, objs = [], []
+    for r, m in INSTANCES:
+        o = os.path.join(OBJ_DIR, f"sweep_kernel_{r}_{m}.o")
+        cmds.append([cc, *HIPCC_FLAGS, "-c", f"-DSTC_INST_REFINE={r}", f"-DSTC_INST_MAXR={m}",
+                     "-o", o, os.path.join(CSRC, "stc_kern_inst.hip")])
+        objs.append(o)
+    for name in ("stc_api", "stc_dispatch"):
+        o = os.path.join(OBJ_DIR, name + ".o")
+        cmds.append([cc, *HIPCC_FLAGS, "-c", "-o", o, os.path.join(CSRC, name + ".hip")])
+        objs.append(o)
+    # the longest compilations first
+    order = sorted(range(len(cmds)), key=lambda i: -int(cmds[i][-3].split("=")[-1])
+                   if cmds[i][-3].startswith("-DSTC_INST_MAXR") else 0)
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(_run, [cmds[i] for i in order]))
+    tmp = OUT + ".tmp"
+    link = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    _run(link)
     os.replace(tmp, OUT)
     # C++ mirror of the reference's sweep entry, over the C ABI
     hcmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", HOST_OUT + ".tmp", *HOST_SRC,
             "-L" + OUT_DIR, "-lstaccato_sweep", "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(hcmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("g++ failed:\n" + r.stdout + r.stderr)
+    _run(hcmd)
     os.replace(HOST_OUT + ".tmp", HOST_OUT)
     if verbose:
-        print(" ".join(cmd))
-        print(" ".join(hcmd))
+        for c in cmds + [link, hcmd]:
+            print(" ".join(c))
     return OUT
 
 

@@ -5,8 +5,13 @@
 // slots.  Every sweep is one persistent launch of sweep_kernel over a device
 // work queue of (system, frequency) points, preceded by a one-lane reset of
 // the queue head (a kernel, so the pair can be captured in a hipGraph).
+//
+// Launches of one handle share its factor slots and queue head, so they are
+// ordered: each waits for the previous one (an event) when it goes to another
+// stream.  A plan owns its own slots and queue head and keeps the ROMs it
+// reads alive (rom_set / rom_clear refuse them while the plan exists).
 #include "../../include/staccato_sweep.h"
-#include "stc_kernels.h"
+#include "stc_launch.h"
 
 #include <algorithm>
 #include <cmath>
@@ -51,6 +56,29 @@ struct RomDev {
     DevBuf K, M, D;
     int nin = 0, nout = 0;
     DevBuf B, C;  // n x nin, nout x n (complex, ld = n / nout)
+    int plan_refs = 0;  // live plans whose graph reads K, M, D
+};
+
+// Orders the launches that share one set of device buffers: a launch on a
+// stream other than the previous one's waits for it first.
+struct LaunchOrder {
+    hipEvent_t ev = nullptr;
+    hipStream_t last = nullptr;
+    bool used = false;
+    int before(hipStream_t st) {
+        if (used && st != last && hipStreamWaitEvent(st, ev, 0) != hipSuccess) return STC_EDEVICE;
+        return STC_OK;
+    }
+    int after(hipStream_t st) {
+        if (hipEventRecord(ev, st) != hipSuccess) return STC_EDEVICE;
+        last = st;
+        used = true;
+        return STC_OK;
+    }
+    // host wait for the last launch (before its buffers are freed or resized)
+    void drain() {
+        if (used) (void)hipEventSynchronize(ev);
+    }
 };
 
 }  // namespace
@@ -62,13 +90,17 @@ struct stc_ctx {
     hipStream_t stream = nullptr;
     std::map<int, RomDev> roms;
     DevBuf scratch, counter;
+    LaunchOrder order;
+    int live_plans = 0;
     // staging for host-memory calls
     DevBuf freq, F, X, info, berr, iters, fac, ipiv, sys, items, adj, c, phi, kd, A;
 };
 
 struct stc_plan_s {
     stc_ctx* h = nullptr;
-    DevBuf sys, items;
+    DevBuf sys, items, scratch, counter;
+    std::vector<int> roms;  // rom ids whose plan_refs this plan holds
+    LaunchOrder order;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -106,7 +138,7 @@ constexpr int kBudgetLarge = 160 * 1024 - 256;  // one workgroup per CU
 // number of workgroups per CU the kernel is built for at which every blocked
 // system still gets a panel of at least 12 columns (a thin panel starves the
 // trailing update).
-int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int jb_cap, int* lds_out) {
+int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int jb_cap, int maxr, int* lds_out) {
     int budget = kBudgetLarge;
     if (const char* e = getenv("STC_JB_MAX")) jb_cap = std::max(4, std::min(atoi(e), jb_cap));
     if (const char* e = getenv("STC_LDS_BUDGET")) {
@@ -117,7 +149,7 @@ int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int jb_cap, int* ld
             if (b == kBudgetThird && wg_per_cu(ts) < 3) continue;
             bool ok = true;
             for (const SysDesc& s : sys) {
-                Layout L = plan_layout(s.n, s.nrhs, ts, b, jb_cap, adj);
+                Layout L = plan_layout(s.n, s.nrhs, ts, b, jb_cap, adj, maxr);
                 if (L.lds > b || (!L.small && L.jb < 12)) ok = false;
             }
             if (ok) {
@@ -128,7 +160,7 @@ int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int jb_cap, int* ld
     }
     int lds = 0;
     for (SysDesc& s : sys) {
-        Layout L = plan_layout(s.n, s.nrhs, ts, budget, jb_cap, adj);
+        Layout L = plan_layout(s.n, s.nrhs, ts, budget, jb_cap, adj, maxr);
         if (L.lds > budget) return STC_EINVAL;
         s.lay_small = L.small;
         s.lay_jb = L.jb;
@@ -140,89 +172,88 @@ int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int jb_cap, int* ld
     return STC_OK;
 }
 
-template <typename T, int R>
-void* kernel_ptr() { return reinterpret_cast<void*>(&sweep_kernel<T, R>); }
-
-template <typename T>
-void* pick_kernel(int maxr) {
-    switch (maxr) {
-        case 1: return kernel_ptr<T, 1>();
-        case 2: return kernel_ptr<T, 2>();
-        case 4: return kernel_ptr<T, 4>();
-        case 8: return kernel_ptr<T, 8>();
-        default: return kernel_ptr<T, 16>();
-    }
-}
-
-template <typename T, int R>
-void launch_typed(dim3 g, int lds, hipStream_t st, const KParams& kp) {
-    hipLaunchKernelGGL((sweep_kernel<T, R>), g, dim3(kThreads), lds, st, kp);
-}
-
-template <typename T>
-void launch_kernel(int maxr, dim3 g, int lds, hipStream_t st, const KParams& kp) {
-    switch (maxr) {
-        case 1: launch_typed<T, 1>(g, lds, st, kp); break;
-        case 2: launch_typed<T, 2>(g, lds, st, kp); break;
-        case 4: launch_typed<T, 4>(g, lds, st, kp); break;
-        case 8: launch_typed<T, 8>(g, lds, st, kp); break;
-        default: launch_typed<T, 16>(g, lds, st, kp); break;
-    }
-}
-
 struct LaunchCfg {
+    int refine = 0;
     int maxr = 1;
     int lds = 0;
     int grid = 1;
     size_t slot_elems = 0;
+    size_t slot_bytes = 0;  // bytes of one factor slot (0: no blocked system)
 };
 
-// Size the launch: grid = resident workgroups (capped by points), scratch slots.
-int configure(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int refine, int adj,
-              int factor_out, LaunchCfg* cfg) {
-    const int ts = refine ? (int)sizeof(cx<float>) : (int)sizeof(cx<double>);
+// Layout and grid of a launch, no allocation: grid = resident workgroups,
+// capped by the points and, when max_slots > 0, by the factor slots available.
+int plan_launch(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int refine, int adj,
+                long long max_slots, LaunchCfg* cfg) {
+    const int ts = refine ? 8 : 16;
     int maxn = 0;
     for (const SysDesc& s : sys) maxn = std::max(maxn, s.n);
     if (maxn < 1 || maxn > 1024) return STC_EINVAL;
     int r = (maxn + 63) / 64, maxr = 1;
     while (maxr < r) maxr <<= 1;
+    cfg->refine = refine;
     cfg->maxr = maxr;
     // the register-resident panel holds 4 (or 2, 1) columns per wavefront
-    const int jb_cap = kWaves * panel_cols_per_wave(refine ? 8 : 16, maxr);
-    int rc = plan_systems(sys, ts, adj, jb_cap, &cfg->lds);
+    const int jb_cap = kWaves * panel_cols_per_wave(ts, maxr);
+    int rc = plan_systems(sys, ts, adj, jb_cap, maxr, &cfg->lds);
     if (rc) return rc;
-    void* kfun = refine ? pick_kernel<float>(maxr) : pick_kernel<double>(maxr);
+    void* kfun = sweep_kernel_fn(refine, maxr);
+    if (!kfun) return STC_EINVAL;
     HIP_TRY(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, cfg->lds));
     int per_cu = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfun, kThreads, cfg->lds));
     if (per_cu < 1) per_cu = 1;
     long long g = (long long)h->num_cu * per_cu;
     if (const char* e = getenv("STC_GRID")) g = std::max(1, atoi(e));
-    cfg->grid = (int)std::max(1LL, std::min(g, nitems));
     // every blocked system factors in a per-workgroup slot (also for stc_getrf,
     // which copies the slot out in LAPACK layout afterwards)
-    (void)factor_out;
     size_t slot = 0;
     for (const SysDesc& s : sys)
         if (!s.lay_small) slot = std::max(slot, slot_elems_for(s.n, s.lay_jb));
     cfg->slot_elems = slot;
-    if (slot) {
-        rc = h->scratch.ensure(slot * ts * (size_t)cfg->grid);
-        if (rc) return rc;
+    cfg->slot_bytes = slot * ts;
+    if (slot && max_slots > 0) g = std::min(g, max_slots);
+    if (slot && max_slots == 0) return STC_EINVAL;
+    cfg->grid = (int)std::max(1LL, std::min(g, nitems));
+    return STC_OK;
+}
+
+// Queue-head reset + the persistent kernel on `st` (nothing else: capturable).
+__global__ void reset_counter_kernel(int* counter) {
+    if (threadIdx.x == 0) counter[0] = 0;
+}
+
+int enqueue_raw(const LaunchCfg& cfg, KParams& kp, int* counter, void* scratch, hipStream_t st) {
+    kp.counter = counter;
+    kp.scratch = scratch;
+    kp.slot_elems = cfg.slot_elems;
+    kp.lds_budget = cfg.lds;
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, st, kp.counter);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(sweep_kernel_launch(cfg.refine, cfg.maxr, cfg.grid, cfg.lds, st, kp));
+    return STC_OK;
+}
+
+// Plan a launch and size the handle's factor slots for it.  Growing the slots
+// frees the old ones, so the handle's previous launch is waited for first.
+int configure(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int refine, int adj,
+              LaunchCfg* cfg) {
+    int rc = plan_launch(h, sys, nitems, refine, adj, -1, cfg);
+    if (rc) return rc;
+    const size_t need = cfg->slot_bytes * (size_t)cfg->grid;
+    if (need > h->scratch.bytes) {
+        h->order.drain();
+        if ((rc = h->scratch.ensure(need))) return rc;
     }
     return STC_OK;
 }
 
-int enqueue(stc_ctx* h, const LaunchCfg& cfg, KParams& kp, int refine, hipStream_t st) {
-    kp.counter = h->counter.as<int>();
-    kp.scratch = h->scratch.p;
-    kp.slot_elems = cfg.slot_elems;
-    kp.lds_budget = cfg.lds;
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, st, kp.counter);
-    if (refine) launch_kernel<float>(cfg.maxr, dim3(cfg.grid), cfg.lds, st, kp);
-    else launch_kernel<double>(cfg.maxr, dim3(cfg.grid), cfg.lds, st, kp);
-    HIP_TRY(hipGetLastError());
-    return STC_OK;
+// A launch on the handle's slots and queue head, ordered after its previous one.
+int enqueue(stc_ctx* h, const LaunchCfg& cfg, KParams& kp, hipStream_t st) {
+    int rc = h->order.before(st);
+    if (rc) return rc;
+    if ((rc = enqueue_raw(cfg, kp, h->counter.as<int>(), h->scratch.p, st))) return rc;
+    return h->order.after(st);
 }
 
 SysDesc base_desc(const RomDev& r) {
@@ -249,16 +280,21 @@ KParams base_params(int mode, double tol, int max_iter) {
 
 int check_mode(int mode) { return (mode == STC_PLAIN_FP64 || mode == STC_REFINED_FP32) ? 0 : 1; }
 
+// info -1: refinement missed tol (soft); info -2: the kernel could not run the
+// point (no factor slot or a layout it cannot hold; the host never plans one),
+// X is NaN there: a hard error.
 int status_from_info(const int* info, long long cnt, stc_stats* st) {
-    int nsing = 0, nconv = 0;
+    int nsing = 0, nconv = 0, nfail = 0;
     for (long long j = 0; j < cnt; ++j) {
         if (info[j] > 0) ++nsing;
-        else if (info[j] < 0) ++nconv;
+        else if (info[j] == -1) ++nconv;
+        else if (info[j] < 0) ++nfail;
     }
     if (st) {
         st->n_singular = nsing;
         st->n_not_converged = nconv;
     }
+    if (nfail) return STC_EDEVICE;
     if (nsing) return STC_ESINGULAR;
     if (nconv) return STC_ENOCONV;
     return STC_OK;
@@ -306,6 +342,7 @@ const char* stc_strerror(int code) {
         case STC_ENOMEM: return "device allocation failed";
         case STC_EDEVICE: return "HIP runtime error";
         case STC_ENOROM: return "rom_id not set";
+        case STC_EBUSY: return "in use by a live plan (destroy the plan first)";
         case STC_ESINGULAR: return "singular A(omega) at one or more frequencies";
         case STC_ENOCONV: return "iterative refinement did not reach tol at one or more frequencies";
         default: return "unknown status";
@@ -333,15 +370,24 @@ int stc_create(stc_handle* out, int device) {
         delete h;
         return STC_ENOMEM;
     }
+    if (hipEventCreateWithFlags(&h->order.ev, hipEventDisableTiming) != hipSuccess) {
+        h->counter.release();
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return STC_EDEVICE;
+    }
     *out = h;
     return STC_OK;
 }
 
 int stc_destroy(stc_handle h) {
     if (!h) return STC_EINVAL;
+    if (h->live_plans > 0) return STC_EBUSY;
     {
         DeviceGuard g(h->device);
+        h->order.drain();
         (void)hipStreamSynchronize(h->stream);
+        (void)hipEventDestroy(h->order.ev);
         for (auto& kv : h->roms) {
             kv.second.K.release(); kv.second.M.release(); kv.second.D.release();
             kv.second.B.release(); kv.second.C.release();
@@ -362,19 +408,31 @@ int stc_rom_set(stc_handle h, int rom_id, int n, const stc_z64* K, const stc_z64
                 const stc_z64* D, int ld, double eta) {
     if (!h || rom_id < 0 || n < 1 || n > 1024 || ld < n || !K || !M || !std::isfinite(eta))
         return STC_EINVAL;
+    const stc_z64* ops[3] = {K, M, D};
+    bool real = true;
+    for (int o = 0; o < 3; ++o) {
+        if (!ops[o]) continue;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const stc_z64 v = ops[o][(size_t)j * ld + i];
+                if (!std::isfinite(v.re) || !std::isfinite(v.im)) return STC_EINVAL;
+                if (v.im != 0.0) real = false;
+            }
+    }
+    auto old = h->roms.find(rom_id);
+    if (old != h->roms.end() && old->second.plan_refs > 0) return STC_EBUSY;
     DeviceGuard g(h->device);
+    // the operators may be read by a launch still in flight
+    h->order.drain();
     RomDev& r = h->roms[rom_id];
+    if (r.n != n) {  // input/output projections belong to the old order
+        r.B.release();
+        r.C.release();
+        r.nin = r.nout = 0;
+    }
     r.n = n;
     r.eta = eta;
     r.has_D = D != nullptr;
-    bool real = true;
-    const stc_z64* ops[3] = {K, M, D};
-    for (int o = 0; o < 3 && real; ++o) {
-        if (!ops[o]) continue;
-        for (int j = 0; j < n && real; ++j)
-            for (int i = 0; i < n; ++i)
-                if (ops[o][(size_t)j * ld + i].im != 0.0) { real = false; break; }
-    }
     r.real_ops = real;
     DevBuf* dst[3] = {&r.K, &r.M, &r.D};
     for (int o = 0; o < 3; ++o) {
@@ -416,7 +474,9 @@ int stc_rom_clear(stc_handle h, int rom_id) {
     if (!h) return STC_EINVAL;
     auto it = h->roms.find(rom_id);
     if (it == h->roms.end()) return STC_ENOROM;
+    if (it->second.plan_refs > 0) return STC_EBUSY;
     DeviceGuard g(h->device);
+    h->order.drain();
     (void)hipStreamSynchronize(h->stream);
     it->second.K.release(); it->second.M.release(); it->second.D.release();
     it->second.B.release(); it->second.C.release();
@@ -449,12 +509,12 @@ int stc_sweep_dev(stc_handle h, int rom_id, int nfreq, const double* d_freq_hz, 
     s.info = d_info;
     s.berr = d_berr;
     LaunchCfg cfg;
-    int rc = configure(h, sys, nfreq, mode == STC_REFINED_FP32, 0, 0, &cfg);
+    int rc = configure(h, sys, nfreq, mode == STC_REFINED_FP32, 0, &cfg);
     if (rc) return rc;
     kp.one = sys[0];
     kp.nitems = nfreq;
     if (d_berr) kp.flags |= kWantBerr;
-    return enqueue(h, cfg, kp, mode == STC_REFINED_FP32, st);
+    return enqueue(h, cfg, kp, st);
 }
 
 int stc_sweep(stc_handle h, int rom_id, int nfreq, const double* freq_hz, int nrhs,
@@ -491,7 +551,7 @@ int stc_sweep(stc_handle h, int rom_id, int nfreq, const double* freq_hz, int nr
     s.berr = berr ? h->berr.as<double>() : nullptr;
     s.iters = h->iters.as<int>();
     LaunchCfg cfg;
-    rc = configure(h, sys, nfreq, mode == STC_REFINED_FP32, 0, 0, &cfg);
+    rc = configure(h, sys, nfreq, mode == STC_REFINED_FP32, 0, &cfg);
     if (rc) return rc;
     kp.one = sys[0];
     kp.nitems = nfreq;
@@ -500,7 +560,7 @@ int stc_sweep(stc_handle h, int rom_id, int nfreq, const double* freq_hz, int nr
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, st));
-    rc = enqueue(h, cfg, kp, mode == STC_REFINED_FP32, st);
+    rc = enqueue(h, cfg, kp, st);
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipStreamSynchronize(st));
     float ms = 0.f;
@@ -561,11 +621,11 @@ int stc_getrf(stc_handle h, int rom_id, int nfreq, const double* freq_hz, int mo
     s.factor_out = h->fac.p;
     s.ipiv_out = h->ipiv.as<int>();
     LaunchCfg cfg;
-    rc = configure(h, sys, nfreq, refine, 0, 1, &cfg);
+    rc = configure(h, sys, nfreq, refine, 0, &cfg);
     if (rc) return rc;
     kp.one = sys[0];
     kp.nitems = nfreq;
-    rc = enqueue(h, cfg, kp, refine, st);
+    rc = enqueue(h, cfg, kp, st);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipMemcpy(LU, h->fac.p, ts * (size_t)n * n * nfreq, hipMemcpyDeviceToHost));
@@ -674,7 +734,7 @@ int stc_sweep_batch(stc_handle h, int nsys, const stc_sys_desc* sysd, int mode, 
     if (items.empty()) return STC_OK;
     const bool refine = mode == STC_REFINED_FP32;
     LaunchCfg cfg;
-    if ((rc = configure(h, sys, (long long)items.size(), refine, 0, 0, &cfg))) return rc;
+    if ((rc = configure(h, sys, (long long)items.size(), refine, 0, &cfg))) return rc;
     if ((rc = upload(h->sys, sys.data(), sys.size() * sizeof(SysDesc), st))) return rc;
     if ((rc = upload(h->items, items.data(), items.size() * sizeof(int2), st))) return rc;
     KParams kp = base_params(mode, tol, max_iter);
@@ -687,7 +747,7 @@ int stc_sweep_batch(stc_handle h, int nsys, const stc_sys_desc* sysd, int mode, 
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, st));
-    rc = enqueue(h, cfg, kp, refine, st);
+    rc = enqueue(h, cfg, kp, st);
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipStreamSynchronize(st));
     float ms = 0.f;
@@ -735,16 +795,24 @@ int stc_plan_create(stc_handle h, int nsys, const stc_sys_desc* dev_sys, int mod
     if (items.empty()) return STC_EINVAL;
     const bool refine = mode == STC_REFINED_FP32;
     LaunchCfg cfg;
-    if ((rc = configure(h, sys, (long long)items.size(), refine, 0, 0, &cfg))) return rc;
+    if ((rc = plan_launch(h, sys, (long long)items.size(), refine, 0, -1, &cfg))) return rc;
     stc_plan_s* p = new stc_plan_s();
     p->h = h;
-    hipStream_t st = h->stream;
-    if ((rc = upload(p->sys, sys.data(), sys.size() * sizeof(SysDesc), st)) ||
-        (rc = upload(p->items, items.data(), items.size() * sizeof(int2), st))) {
+    auto fail = [&](int code) {
+        if (p->order.ev) (void)hipEventDestroy(p->order.ev);
+        p->sys.release(); p->items.release(); p->scratch.release(); p->counter.release();
         delete p;
-        return rc;
-    }
-    if (hipStreamSynchronize(st) != hipSuccess) { delete p; return STC_EDEVICE; }
+        return code;
+    };
+    hipStream_t st = h->stream;
+    // the plan's own factor slots and queue head: no later call on the handle
+    // can resize or share them
+    if ((rc = p->counter.ensure(256)) || (cfg.slot_bytes && (rc = p->scratch.ensure(cfg.slot_bytes * cfg.grid))) ||
+        (rc = upload(p->sys, sys.data(), sys.size() * sizeof(SysDesc), st)) ||
+        (rc = upload(p->items, items.data(), items.size() * sizeof(int2), st)))
+        return fail(rc);
+    if (hipEventCreateWithFlags(&p->order.ev, hipEventDisableTiming) != hipSuccess) return fail(STC_EDEVICE);
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(STC_EDEVICE);
     KParams kp = base_params(mode, tol, max_iter);
     kp.sys = p->sys.as<SysDesc>();
     kp.nsys = nsys;
@@ -752,26 +820,29 @@ int stc_plan_create(stc_handle h, int nsys, const stc_sys_desc* dev_sys, int mod
     kp.nitems = (int)items.size();
     for (const SysDesc& s : sys)
         if (s.berr) kp.flags |= kWantBerr;
-    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        delete p;
-        return STC_EDEVICE;
-    }
-    rc = enqueue(h, cfg, kp, refine, st);
+    // the handle's previous launch is not part of the graph
+    h->order.drain();
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail(STC_EDEVICE);
+    rc = enqueue_raw(cfg, kp, p->counter.as<int>(), p->scratch.p, st);
     hipGraph_t graph = nullptr;
     const hipError_t ce = hipStreamEndCapture(st, &graph);
     if (rc || ce != hipSuccess || !graph) {
         if (graph) (void)hipGraphDestroy(graph);
-        p->sys.release(); p->items.release();
-        delete p;
-        return rc ? rc : STC_EDEVICE;
+        return fail(rc ? rc : STC_EDEVICE);
     }
     p->graph = graph;
     if (hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
         (void)hipGraphDestroy(graph);
-        p->sys.release(); p->items.release();
-        delete p;
-        return STC_EDEVICE;
+        p->graph = nullptr;
+        return fail(STC_EDEVICE);
     }
+    // the graph reads these ROMs' operators: keep them until the plan goes
+    for (int i = 0; i < nsys; ++i) {
+        if (std::find(p->roms.begin(), p->roms.end(), dev_sys[i].rom_id) != p->roms.end()) continue;
+        p->roms.push_back(dev_sys[i].rom_id);
+        h->roms[dev_sys[i].rom_id].plan_refs++;
+    }
+    h->live_plans++;
     *out = p;
     return STC_OK;
 }
@@ -780,19 +851,31 @@ int stc_plan_execute(stc_plan p, void* stream) {
     if (!p || !p->exec) return STC_EINVAL;
     DeviceGuard g(p->h->device);
     hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : p->h->stream;
+    // replays share the plan's slots and queue head: order them
+    int rc = p->order.before(st);
+    if (rc) return rc;
     HIP_TRY(hipGraphLaunch(p->exec, st));
-    return STC_OK;
+    return p->order.after(st);
 }
 
 int stc_plan_destroy(stc_plan p) {
     if (!p) return STC_EINVAL;
     {
         DeviceGuard g(p->h->device);
+        p->order.drain();
         (void)hipStreamSynchronize(p->h->stream);
         if (p->exec) (void)hipGraphExecDestroy(p->exec);
         if (p->graph) (void)hipGraphDestroy(p->graph);
+        (void)hipEventDestroy(p->order.ev);
         p->sys.release();
         p->items.release();
+        p->scratch.release();
+        p->counter.release();
+        for (int id : p->roms) {
+            auto it = p->h->roms.find(id);
+            if (it != p->h->roms.end() && it->second.plan_refs > 0) it->second.plan_refs--;
+        }
+        p->h->live_plans--;
     }
     delete p;
     return STC_OK;
@@ -837,10 +920,10 @@ int stc_adjoint(stc_handle h, int rom_id, int nfreq, const double* freq_hz, int 
     s.adj_out = h->adj.as<double>();
     const bool refine = mode == STC_REFINED_FP32;
     LaunchCfg cfg;
-    if ((rc = configure(h, sys, nfreq, refine, 1, 0, &cfg))) return rc;
+    if ((rc = configure(h, sys, nfreq, refine, 1, &cfg))) return rc;
     kp.one = sys[0];
     kp.nitems = nfreq;
-    if ((rc = enqueue(h, cfg, kp, refine, st))) return rc;
+    if ((rc = enqueue(h, cfg, kp, st))) return rc;
     HIP_TRY(hipStreamSynchronize(st));
     std::vector<double> ao((size_t)nfreq * (p + 2));
     HIP_TRY(hipMemcpy(ao.data(), h->adj.p, ao.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -908,14 +991,14 @@ int stc_back_transform(stc_handle h, int rom_id, int nfreq, const double* freq_h
     s.X = h->X.as<double2>();
     s.info = h->info.as<int>();
     LaunchCfg cfg;
-    if ((rc = configure(h, sys, nfreq, mode == STC_REFINED_FP32, 0, 0, &cfg))) return rc;
+    if ((rc = configure(h, sys, nfreq, mode == STC_REFINED_FP32, 0, &cfg))) return rc;
     kp.one = sys[0];
     kp.nitems = nfreq;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, st));
-    if ((rc = enqueue(h, cfg, kp, mode == STC_REFINED_FP32, st))) return rc;
+    if ((rc = enqueue(h, cfg, kp, st))) return rc;
     // outputs y = C_R X and H_R = C_R A^{-1} B_R for every frequency (:1438-1448)
     double2* Yd = h->berr.as<double2>();
     if ((size_t)nfreq * nout * ncol * sizeof(double2) > h->berr.bytes) return STC_EINVAL;
@@ -981,12 +1064,12 @@ int stc_gesv_batched_dev(stc_handle h, int n, int batch, const stc_z64* d_A, lon
     s.info = d_info;
     s.berr = d_berr;
     LaunchCfg cfg;
-    int rc = configure(h, sys, batch, mode == STC_REFINED_FP32, 0, 0, &cfg);
+    int rc = configure(h, sys, batch, mode == STC_REFINED_FP32, 0, &cfg);
     if (rc) return rc;
     kp.one = sys[0];
     kp.nitems = batch;
     if (d_berr) kp.flags |= kWantBerr;
-    return enqueue(h, cfg, kp, mode == STC_REFINED_FP32, st);
+    return enqueue(h, cfg, kp, st);
 }
 
 int stc_gesv_batched(stc_handle h, int n, int batch, const stc_z64* A, int lda, long long strideA,

@@ -129,9 +129,19 @@ __host__ __device__ inline int lds_solve_bytes(int n, int G, int ts, int adj) {
     return align16((long long)n * G * ts) + (adj ? 4 : 2) * align16((long long)n * G * 16);
 }
 
+// Complex elements of the blocked path's LDS panel region: the n x jb panel,
+// which also holds the two alternating multiplier columns of the register
+// panel (64 * maxr elements each, maxr = rows per lane of the launch, which a
+// small system of a mixed batch shares with the largest one).
+__host__ __device__ inline long long panel_region_elems(int n, int jb, int maxr) {
+    const long long a = (long long)n * jb, b = 2LL * 64 * maxr;
+    return a > b ? a : b;
+}
+
 // Choose the path and blocking for a system of order n.  ts = sizeof(complex
-// factor element) (8 or 16).  budget = dynamic LDS bytes of the launch.
-inline Layout plan_layout(int n, int nrhs, int ts, int budget, int jb_cap, int adj) {
+// factor element) (8 or 16).  budget = dynamic LDS bytes of the launch; maxr =
+// rows per lane of the kernel the launch uses.
+inline Layout plan_layout(int n, int nrhs, int ts, int budget, int jb_cap, int adj, int maxr) {
     Layout L{0, 0, 0, 1, 0};
     const int fixed = lds_fixed_bytes(n);
     int Gmax = adj ? 1 : (nrhs < kMaxGroup ? nrhs : kMaxGroup);
@@ -150,8 +160,8 @@ inline Layout plan_layout(int n, int nrhs, int ts, int budget, int jb_cap, int a
     for (int a = 0; a < 8; ++a) {
         int jb = jbs[a];
         if (jb > jb_cap) continue;
-        int need = fixed + align16((long long)n * jb * ts) + align16((long long)kWaves * jb * kUwLdCols * ts) +
-                   align16((long long)jb * jb * ts);
+        int need = fixed + align16(panel_region_elems(n, jb, maxr) * ts) +
+                   align16((long long)kWaves * jb * kUwLdCols * ts) + align16((long long)jb * jb * ts);
         if (need <= budget) {
             L.jb = jb; L.cw = kUwCols; L.lds = need;
             for (int G = Gmax; G >= 1; --G) {

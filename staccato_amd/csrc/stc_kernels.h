@@ -25,6 +25,8 @@
 
 #include "stc_internal.h"
 
+#include <type_traits>
+
 namespace stc {
 
 template <typename T>
@@ -175,6 +177,18 @@ __device__ __forceinline__ z64 assemble_at(const SysDesc& s, size_t e, double w,
                        k.im + s.eta * k.re - w2 * m.im + w * d.re);
 }
 
+// ---- pivot key of a candidate row: |re| + |im|, with LAPACK izamax's NaN rule ------
+// izamax starts from the first element and replaces it only by a strictly
+// larger one, so a NaN in the first eligible row keeps that row, and a NaN
+// anywhere else is never chosen.  As a key: NaN in the first row -> +inf (it
+// wins every comparison, ties go to the first row), NaN elsewhere -> -0.5
+// (below every number, above the "no candidate" start value -1).
+template <typename T>
+__device__ __forceinline__ T pivot_key(cx<T> a, bool first) {
+    const T v = cabs1(a);
+    return v == v ? v : (first ? T(__builtin_inf()) : T(-0.5));
+}
+
 // ---- pivot search: first index of max |re|+|im| over rows [start, m) -------------
 // Executed by one whole wavefront; every lane returns the same row.
 template <typename T>
@@ -183,7 +197,7 @@ __device__ __forceinline__ int wave_argmax(const cx<T>* col, int start, int m) {
     T best = T(-1);
     int bi = 0x7fffffff;
     for (int i = start + lane; i < m; i += 64) {
-        T v = cabs1(col[i]);
+        T v = pivot_key(col[i], i == start);
         if (v > best) { best = v; bi = i; }
     }
 #pragma unroll
@@ -444,13 +458,14 @@ __device__ __forceinline__ void panel_factor_reg(cx<T>* __restrict__ S, int ld, 
         if (wave == (jj & (kWaves - 1))) {
             // argmax of |re|+|im| over rows with lpos >= jj; ties: smallest lpos.
             // All-zero column: the smallest eligible lpos is jj itself, so the
-            // row stays (LAPACK: ipiv = jj) and no scaling happens.
+            // row stays (LAPACK: ipiv = jj) and no scaling happens.  NaN keys
+            // follow izamax (pivot_key), so some eligible row always wins.
             T best = T(-1);
             int bpos = 0x7fffffff, bg = 0;
 #pragma unroll
             for (int t = 0; t < MAXR; ++t) {
                 if (lpos[t] >= jj && lpos[t] != 0x7fffffff) {
-                    const T v = cabs1(x[0][t]);
+                    const T v = pivot_key(x[0][t], lpos[t] == jj);
                     if (v > best || (v == best && lpos[t] < bpos)) {
                         best = v;
                         bpos = lpos[t];
@@ -791,7 +806,7 @@ __device__ __forceinline__ void factor_blocked(const SysDesc& s, double w, doubl
 
     const int jbmax = lay.jb;
     cx<T>* P = reinterpret_cast<cx<T>*>(L.main);
-    cx<T>* Uw = reinterpret_cast<cx<T>*>(L.main + align16((long long)n * jbmax * sizeof(cx<T>)));
+    cx<T>* Uw = reinterpret_cast<cx<T>*>(L.main + align16(panel_region_elems(n, jbmax, MAXR) * sizeof(cx<T>)));
     cx<T>* Linv = reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(Uw) +
                                           align16((long long)kWaves * jbmax * kUwLd * sizeof(cx<T>)));
     cx<T>* lcol = P;  // the multiplier column lives in the panel region while the panel is in registers
@@ -1435,9 +1450,15 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
         solvep = L.main + align16((long long)nn * sizeof(cx<T>));
     } else {
         if (kp.scratch == nullptr || kp.slot_elems < slot_elems_for(n, lay.jb) ||
-            lay.jb > kWaves * panel_cols_per_wave((int)sizeof(cx<T>), MAXR) || lay.jb > 32 ||
-            2 * 64 * MAXR > n * lay.jb) {
-            // no factor slot, or a panel wider than the register panel: report, touch nothing
+            lay.jb > kWaves * panel_cols_per_wave((int)sizeof(cx<T>), MAXR) || lay.jb > 32) {
+            // no factor slot, or a panel wider than the register panel (the host
+            // plans neither): report, write NaN, touch nothing else
+            double2* Xn = s.X ? s.X + (size_t)fidx * s.ldx * nrhs : nullptr;
+            if (Xn)
+                for (int e = tid; e < n * nrhs; e += kThreads) {
+                    const int r = e % n, g = e / n;
+                    Xn[r + (size_t)g * s.ldx] = make_double2(__builtin_nan(""), __builtin_nan(""));
+                }
             if (tid == 0) s.info[fidx] = -2;
             __syncthreads();
             return;
@@ -1597,10 +1618,6 @@ sweep_kernel(KParams kp) {
         process_point<T, MAXR>(s, fidx, kp, smem);
         __syncthreads();
     }
-}
-
-__global__ void reset_counter_kernel(int* counter) {
-    if (threadIdx.x == 0) counter[0] = 0;
 }
 
 }  // namespace stc

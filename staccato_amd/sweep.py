@@ -53,6 +53,23 @@ def _stream_arg(stream):
     return ctypes.c_void_p(int(stream))
 
 
+def _check_tensor(t, name, dtype, shape, device=None):
+    """Refuse a tensor the C ABI would read or write out of bounds: wrong
+    dtype, shape, layout (the ABI takes packed column-major blocks) or device."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch.Tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {dtype}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+    if t.device.type != "cuda" or (device is not None and t.device.index != device):
+        raise ValueError(f"{name}: must live on cuda:{device}, not {t.device}")
+    return t
+
+
 def _check(rc, what, soft=(0,)):
     if rc not in soft:
         raise StcError(rc, what)
@@ -133,8 +150,18 @@ class Handle:
         """Device-resident sweep on torch tensors (no copies, no synchronisation).
         F_t: (nrhs, n) complex128 contiguous (column-major n x nrhs), X_t:
         (nfreq, nrhs, n) complex128, info_t int32 (nfreq), berr_t float64 or None."""
+        import torch
         n = self.n[rom_id]
+        if F_t.dim() != 2:
+            raise ValueError("F_t: expected (nrhs, n)")
         nrhs = F_t.shape[0]
+        nf = freq_t.numel()
+        _check_tensor(freq_t, "freq_t", torch.float64, (nf,), self.device)
+        _check_tensor(F_t, "F_t", torch.complex128, (nrhs, n), self.device)
+        _check_tensor(X_t, "X_t", torch.complex128, (nf, nrhs, n), self.device)
+        _check_tensor(info_t, "info_t", torch.int32, (nf,), self.device)
+        if berr_t is not None:
+            _check_tensor(berr_t, "berr_t", torch.float64, (nf, nrhs), self.device)
         s = _stream_arg(stream)
         rc = self.lib.stc_sweep_dev(
             self.h, rom_id, freq_t.numel(), ctypes.c_void_p(freq_t.data_ptr()), nrhs,
@@ -250,8 +277,17 @@ class Handle:
         synchronisation).  A_t: (batch, n, n) complex128 holding each matrix
         column-major (A[j] transposed in memory), B_t: (batch, nrhs, n)
         column-major blocks, X_t: (batch, nrhs, n), info_t int32 (batch)."""
+        import torch
+        if A_t.dim() != 3 or B_t.dim() != 3:
+            raise ValueError("A_t: expected (batch, n, n), B_t: (batch, nrhs, n)")
         nb, n = A_t.shape[0], A_t.shape[1]
         nrhs = B_t.shape[1]
+        _check_tensor(A_t, "A_t", torch.complex128, (nb, n, n), self.device)
+        _check_tensor(B_t, "B_t", torch.complex128, (nb, nrhs, n), self.device)
+        _check_tensor(X_t, "X_t", torch.complex128, (nb, nrhs, n), self.device)
+        _check_tensor(info_t, "info_t", torch.int32, (nb,), self.device)
+        if berr_t is not None:
+            _check_tensor(berr_t, "berr_t", torch.float64, (nb, nrhs), self.device)
         s = _stream_arg(stream)
         rc = self.lib.stc_gesv_batched_dev(
             self.h, n, nb, ctypes.c_void_p(A_t.data_ptr()), n * n, nrhs,

@@ -4,7 +4,9 @@ The reference holds no fixtures or golden vectors for the sweep (its checks
 are MATLAB scripts over data it does not ship, srcPerformanceTest/
 checkSolution_60.m:8-69).  The arithmetic it delegates to is LAPACK
 zgetrf/zgetrs (KrylovROMSubstructure.cpp:1428-1430), so the fixtures come from
-an independent LAPACK: the OpenBLAS bundled with scipy (scipy.linalg.lapack).
+an independent LAPACK: the OpenBLAS bundled with scipy (scipy.linalg.lapack):
+zgetrf, zgetrs 'N'/'T'/'C', and their single-precision counterparts cgetrf
+and cgetrs 'N'/'T'/'C' (the factor and solves of the refined mode).
 
     python tests/golden/gen_golden.py     -> tests/golden/lapack_small.npz
 """
@@ -37,6 +39,11 @@ def main():
         assert info == 0
         out[f"LU32_{n}"] = lu32
         out[f"ipiv32_{n}"] = (piv32 + 1).astype(np.int32)
+        # cgetrs on the cgetrf factors, 'N' / 'T' / 'C'
+        for t, code in (("N", 0), ("T", 1), ("C", 2)):
+            x, info = lapack.cgetrs(lu32, piv32, B.astype(np.complex64), trans=code)
+            assert info == 0
+            out[f"X32{t}_{n}"] = x
     # an exactly zero second column: U(2,2) = 0 in any rounding, zgetrf info = 2
     S = np.array([[1.0, 0.0, 3.0], [2.0, 0.0, 1.0], [3.0, 0.0, 2.0]], dtype=np.complex128)
     lu, piv, info = lapack.zgetrf(S)

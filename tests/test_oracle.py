@@ -3,8 +3,10 @@
 The reference ships no fixtures for the sweep, so each oracle function is
 checked against something independent of its own formula:
   - LAPACK golden vectors written by tests/golden/gen_golden.py (scipy's
-    OpenBLAS zgetrf/zgetrs/cgetrf): pivots, factors, 'N'/'T'/'C' solves,
-    the zero-pivot info;
+    OpenBLAS zgetrf/zgetrs/cgetrf/cgetrs): pivots, factors, 'N'/'T'/'C'
+    solves in both precisions, the zero-pivot info;
+  - the normwise backward error against an independent long-double
+    evaluation (tests/berr_check.py);
   - brute force for n = 1, 2, 3 (Cramer's rule / explicit inverse);
   - the closed-form modal solution X = Phi diag(1/lam) Phi^T F of the
     synthetic ROMs, and the exact condition number from lam;
@@ -50,6 +52,48 @@ def test_cgetrf_matches_lapack(n):
     np.testing.assert_array_equal(ipiv, GOLD[f"ipiv32_{n}"])
     ref = GOLD[f"LU32_{n}"]
     assert np.abs(LU - ref).max() <= 1e-5 * max(1.0, np.abs(ref).max()) * n
+
+
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("trans", ["N", "T", "C"])
+def test_cgetrs_matches_lapack(n, trans):
+    """cgetrs on the cgetrf fixture factors against LAPACK's cgetrs."""
+    X = oracle.cgetrs(GOLD[f"LU32_{n}"], GOLD[f"ipiv32_{n}"], GOLD[f"B_{n}"], trans)
+    ref = GOLD[f"X32{trans}_{n}"]
+    assert np.abs(X - ref).max() <= 2e-6 * max(1.0, np.abs(ref).max()) * n
+
+
+def test_cgetrs_rejects_planted_transpose():
+    """The fixture tells 'T' from 'C' (a dropped conjugation fails)."""
+    n = 8
+    X = oracle.cgetrs(GOLD[f"LU32_{n}"], GOLD[f"ipiv32_{n}"], GOLD[f"B_{n}"], "T")
+    ref = GOLD[f"X32C_{n}"]
+    assert np.abs(X - ref).max() > 1e-2 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("n", [1, 3, 17, 60])
+def test_backward_error_matches_extended_precision(n):
+    """orc_backward_error against the long-double evaluation of
+    ||f - A x||_inf / (||A||_inf ||x||_inf + ||f||_inf) (tests/berr_check.py),
+    on solutions perturbed to berr ~ 1e-8, and planted mistakes in the
+    long-double checker (1-norm of the vectors, |re|+|im| instead of the
+    modulus, ||f|| dropped) disagree with the oracle."""
+    from berr_check import berr
+    rng = np.random.default_rng(40 + n)
+    for trial in range(5):
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        f = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        x = np.linalg.solve(A, f)
+        x = x * (1 + 1e-8 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)))
+        ex = float(berr(A, x, f)[0])
+        got = oracle.backward_error(A, x, f)
+        assert abs(got - ex) <= 2 * (n + 2) * 2.0 ** -53 + 1e-9 * ex, (got, ex)
+        assert ex > 1e-11
+        for kw in ({"vec_norm": "one"}, {"elem": "cabs1"}, {"drop_f": True}):
+            if n == 1 and "vec_norm" in kw:
+                continue     # one entry: both vector norms agree
+            bad = float(berr(A, x, f, **kw)[0])
+            assert abs(bad - got) > 1e-6 * ex, kw
 
 
 def test_zero_pivot_info_and_ipiv():
