@@ -73,6 +73,8 @@ enum {
     STC_EDEVICE = -3,  /* HIP runtime error                               */
     STC_ENOROM = -4,   /* rom_id not set on this handle                   */
     STC_EBUSY = -5,    /* rom or handle still used by a live plan         */
+    STC_ETOOBIG = -6,  /* system too large for the fixed scratch of
+                          stc_sweep_dev_noalloc                           */
     STC_ESINGULAR = 1, /* at least one frequency had info > 0             */
     STC_ENOCONV = 2    /* at least one frequency had info == -1           */
 };
@@ -149,6 +151,25 @@ int stc_sweep_dev(stc_handle h, int rom_id, int nfreq, const double* d_freq_hz,
                   int nrhs, const stc_z64* d_F, int ldf, stc_z64* d_X, int ldx,
                   int* d_info, double* d_berr, int mode, double tol,
                   int max_iter, void* stream);
+
+/* Allocation-free variant of stc_sweep_dev, same arguments and results.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory during the call (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc / hipFree and no
+ * allocating library routine: the factor slots and the work-queue head live
+ * in a fixed device array of the library (64 MiB per GPU, held from the
+ * library's first stc_create on that GPU until the process ends).  Points run
+ * on as many workgroups as that array has factor slots for, so very large n
+ * runs on fewer workgroups (n = 1024 in complex128: three); a system whose
+ * single slot would exceed the array is rejected with STC_ETOOBIG.  Every call on one GPU shares that GPU's array, whatever the handle:
+ * calls that overlap in time on one GPU (other streams, other handles, other
+ * host threads) race and corrupt each other's results; order them.  The queue
+ * head is cleared by a kernel at the start of each call. */
+int stc_sweep_dev_noalloc(stc_handle h, int rom_id, int nfreq,
+                          const double* d_freq_hz, int nrhs, const stc_z64* d_F,
+                          int ldf, stc_z64* d_X, int ldx, int* d_info,
+                          double* d_berr, int mode, double tol, int max_iter,
+                          void* stream);
 
 /* ---- LU factors (LAPACK zgetrf / cgetrf layout) ---------------------------------
  * Factor A(2 pi freq_hz[j]) for nfreq frequencies and return the in-place

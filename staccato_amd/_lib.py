@@ -16,6 +16,7 @@ STC_ENOMEM = -2
 STC_EDEVICE = -3
 STC_ENOROM = -4
 STC_EBUSY = -5
+STC_ETOOBIG = -6
 STC_ESINGULAR = 1
 STC_ENOCONV = 2
 
@@ -26,7 +27,7 @@ STC_REFINED_FP32 = 1
 EXPORTED = (
     "stc_create", "stc_destroy", "stc_strerror", "stc_device_ok",
     "stc_rom_set", "stc_rom_set_io", "stc_rom_clear",
-    "stc_sweep", "stc_sweep_dev", "stc_getrf",
+    "stc_sweep", "stc_sweep_dev", "stc_sweep_dev_noalloc", "stc_getrf",
     "stc_sweep_batch", "stc_plan_create", "stc_plan_execute", "stc_plan_destroy",
     "stc_adjoint", "stc_back_transform", "stc_gesv_batched", "stc_gesv_batched_dev",
 )
@@ -92,6 +93,7 @@ def load(path=None):
         "stc_rom_clear": (i, [vp, i]),
         "stc_sweep": (i, [vp, i, i, vp, i, vp, i, vp, i, vp, vp, i, d, i, vp]),
         "stc_sweep_dev": (i, [vp, i, i, vp, i, vp, i, vp, i, vp, vp, i, d, i, vp]),
+        "stc_sweep_dev_noalloc": (i, [vp, i, i, vp, i, vp, i, vp, i, vp, vp, i, d, i, vp]),
         "stc_getrf": (i, [vp, i, i, vp, i, vp, vp, vp]),
         "stc_sweep_batch": (i, [vp, i, vp, i, d, i, vp]),
         "stc_plan_create": (i, [vp, i, vp, i, d, i, ctypes.POINTER(vp)]),

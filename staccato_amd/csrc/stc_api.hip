@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -87,6 +88,7 @@ struct stc_ctx {
     int device = 0;
     int num_cu = 0;
     int ok_arch = 0;
+    void* na_scratch = nullptr;  // device address of this GPU's `scratch` array (noalloc entry)
     hipStream_t stream = nullptr;
     std::map<int, RomDev> roms;
     DevBuf scratch, counter;
@@ -130,6 +132,32 @@ int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
     return STC_OK;
 }
 
+// ---- fixed scratch of the allocation-free entry (stc_sweep_dev_noalloc) -----------------
+// One array per GPU, part of the library's code object: the queue head in its
+// first 256 bytes, the factor slots after it.  In host code `scratch` names a
+// host-side stand-in, never the device memory: the device address comes from
+// hipGetSymbolAddress, once per GPU (at its first stc_create, with that GPU
+// current), and only that address is passed to kernels.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+constexpr size_t kNoAllocHead = 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[kMaxDevices] = {};
+
+// Device address of the current GPU's `scratch` (device == current ordinal).
+void* noalloc_scratch_of(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_na_ptr[device] = p;
+    }
+    return g_na_ptr[device];
+}
+
 constexpr int kBudgetThird = 53 * 1024;       // three workgroups per CU
 constexpr int kBudgetSmall = 80 * 1024;       // two workgroups per CU
 constexpr int kBudgetLarge = 160 * 1024 - 256;  // one workgroup per CU
@@ -138,7 +166,7 @@ constexpr int kBudgetLarge = 160 * 1024 - 256;  // one workgroup per CU
 // number of workgroups per CU the kernel is built for at which every blocked
 // system still gets a panel of at least 12 columns (a thin panel starves the
 // trailing update).
-int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int jb_cap, int maxr, int* lds_out) {
+int plan_systems(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, int* lds_out) {
     int budget = kBudgetLarge;
     if (const char* e = getenv("STC_JB_MAX")) jb_cap = std::max(4, std::min(atoi(e), jb_cap));
     if (const char* e = getenv("STC_LDS_BUDGET")) {
@@ -148,7 +176,8 @@ int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int jb_cap, int max
         for (int b : cands) {
             if (b == kBudgetThird && wg_per_cu(ts) < 3) continue;
             bool ok = true;
-            for (const SysDesc& s : sys) {
+            for (int i = 0; i < nsys; ++i) {
+                const SysDesc& s = sys[i];
                 Layout L = plan_layout(s.n, s.nrhs, ts, b, jb_cap, adj, maxr);
                 if (L.lds > b || (!L.small && L.jb < 12)) ok = false;
             }
@@ -159,7 +188,8 @@ int plan_systems(std::vector<SysDesc>& sys, int ts, int adj, int jb_cap, int max
         }
     }
     int lds = 0;
-    for (SysDesc& s : sys) {
+    for (int i = 0; i < nsys; ++i) {
+        SysDesc& s = sys[i];
         Layout L = plan_layout(s.n, s.nrhs, ts, budget, jb_cap, adj, maxr);
         if (L.lds > budget) return STC_EINVAL;
         s.lay_small = L.small;
@@ -181,13 +211,14 @@ struct LaunchCfg {
     size_t slot_bytes = 0;  // bytes of one factor slot (0: no blocked system)
 };
 
-// Layout and grid of a launch, no allocation: grid = resident workgroups,
-// capped by the points and, when max_slots > 0, by the factor slots available.
-int plan_launch(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int refine, int adj,
-                long long max_slots, LaunchCfg* cfg) {
+// Layout and grid of a launch, no allocation (host or device): grid = resident
+// workgroups, capped by the points and, when slot_budget > 0, by the factor
+// slots that fit in slot_budget bytes (STC_ETOOBIG when not even one does).
+int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine, int adj,
+                size_t slot_budget, LaunchCfg* cfg) {
     const int ts = refine ? 8 : 16;
     int maxn = 0;
-    for (const SysDesc& s : sys) maxn = std::max(maxn, s.n);
+    for (int i = 0; i < nsys; ++i) maxn = std::max(maxn, sys[i].n);
     if (maxn < 1 || maxn > 1024) return STC_EINVAL;
     int r = (maxn + 63) / 64, maxr = 1;
     while (maxr < r) maxr <<= 1;
@@ -195,7 +226,7 @@ int plan_launch(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int ref
     cfg->maxr = maxr;
     // the register-resident panel holds 4 (or 2, 1) columns per wavefront
     const int jb_cap = kWaves * panel_cols_per_wave(ts, maxr);
-    int rc = plan_systems(sys, ts, adj, jb_cap, maxr, &cfg->lds);
+    int rc = plan_systems(sys, nsys, ts, adj, jb_cap, maxr, &cfg->lds);
     if (rc) return rc;
     void* kfun = sweep_kernel_fn(refine, maxr);
     if (!kfun) return STC_EINVAL;
@@ -208,12 +239,16 @@ int plan_launch(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int ref
     // every blocked system factors in a per-workgroup slot (also for stc_getrf,
     // which copies the slot out in LAPACK layout afterwards)
     size_t slot = 0;
-    for (const SysDesc& s : sys)
-        if (!s.lay_small) slot = std::max(slot, slot_elems_for(s.n, s.lay_jb));
+    for (int i = 0; i < nsys; ++i)
+        if (!sys[i].lay_small)
+            slot = std::max(slot, slot_elems_for(sys[i].n, sys[i].lay_jb));
     cfg->slot_elems = slot;
     cfg->slot_bytes = slot * ts;
-    if (slot && max_slots > 0) g = std::min(g, max_slots);
-    if (slot && max_slots == 0) return STC_EINVAL;
+    if (slot && slot_budget > 0) {
+        const long long fit = (long long)(slot_budget / cfg->slot_bytes);
+        if (fit < 1) return STC_ETOOBIG;
+        g = std::min(g, fit);
+    }
     cfg->grid = (int)std::max(1LL, std::min(g, nitems));
     return STC_OK;
 }
@@ -238,7 +273,7 @@ int enqueue_raw(const LaunchCfg& cfg, KParams& kp, int* counter, void* scratch, 
 // frees the old ones, so the handle's previous launch is waited for first.
 int configure(stc_ctx* h, std::vector<SysDesc>& sys, long long nitems, int refine, int adj,
               LaunchCfg* cfg) {
-    int rc = plan_launch(h, sys, nitems, refine, adj, -1, cfg);
+    int rc = plan_launch(h, sys.data(), (int)sys.size(), nitems, refine, adj, 0, cfg);
     if (rc) return rc;
     const size_t need = cfg->slot_bytes * (size_t)cfg->grid;
     if (need > h->scratch.bytes) {
@@ -343,6 +378,7 @@ const char* stc_strerror(int code) {
         case STC_EDEVICE: return "HIP runtime error";
         case STC_ENOROM: return "rom_id not set";
         case STC_EBUSY: return "in use by a live plan (destroy the plan first)";
+        case STC_ETOOBIG: return "system too large for the 64 MiB scratch of stc_sweep_dev_noalloc";
         case STC_ESINGULAR: return "singular A(omega) at one or more frequencies";
         case STC_ENOCONV: return "iterative refinement did not reach tol at one or more frequencies";
         default: return "unknown status";
@@ -376,6 +412,9 @@ int stc_create(stc_handle* out, int device) {
         delete h;
         return STC_EDEVICE;
     }
+    // set-up of the allocation-free entry (null when unavailable: that entry
+    // then returns STC_EDEVICE; every other call works without it)
+    if (h->ok_arch) h->na_scratch = noalloc_scratch_of(device);
     *out = h;
     return STC_OK;
 }
@@ -515,6 +554,43 @@ int stc_sweep_dev(stc_handle h, int rom_id, int nfreq, const double* d_freq_hz, 
     kp.nitems = nfreq;
     if (d_berr) kp.flags |= kWantBerr;
     return enqueue(h, cfg, kp, st);
+}
+
+// Same launch as stc_sweep_dev on the GPU's fixed `scratch` array (through its
+// device address, h->na_scratch): no device or host allocation, no handle
+// buffer, no ordering with the handle's other launches.
+int stc_sweep_dev_noalloc(stc_handle h, int rom_id, int nfreq, const double* d_freq_hz, int nrhs,
+                          const stc_z64* d_F, int ldf, stc_z64* d_X, int ldx, int* d_info,
+                          double* d_berr, int mode, double tol, int max_iter, void* stream) {
+    if (!h || nfreq < 0 || nrhs < 1 || check_mode(mode)) return STC_EINVAL;
+    auto it = h->roms.find(rom_id);
+    if (it == h->roms.end()) return STC_ENOROM;
+    const RomDev& r = it->second;
+    if (ldf < r.n || ldx < r.n || (nfreq && (!d_freq_hz || !d_F || !d_X || !d_info)))
+        return STC_EINVAL;
+    if (nfreq == 0) return STC_OK;
+    if (!h->na_scratch) return STC_EDEVICE;
+    DeviceGuard g(h->device);
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
+    KParams kp = base_params(mode, tol, max_iter);
+    SysDesc s = base_desc(r);
+    s.nrhs = nrhs;
+    s.nfreq = nfreq;
+    s.ldf = ldf;
+    s.ldx = ldx;
+    s.freq = d_freq_hz;
+    s.F = reinterpret_cast<const double2*>(d_F);
+    s.X = reinterpret_cast<double2*>(d_X);
+    s.info = d_info;
+    s.berr = d_berr;
+    LaunchCfg cfg;
+    int rc = plan_launch(h, &s, 1, nfreq, mode == STC_REFINED_FP32, 0, kNoAllocBytes - kNoAllocHead, &cfg);
+    if (rc) return rc;
+    kp.one = s;
+    kp.nitems = nfreq;
+    if (d_berr) kp.flags |= kWantBerr;
+    char* base = static_cast<char*>(h->na_scratch);
+    return enqueue_raw(cfg, kp, reinterpret_cast<int*>(base), base + kNoAllocHead, st);
 }
 
 int stc_sweep(stc_handle h, int rom_id, int nfreq, const double* freq_hz, int nrhs,
@@ -795,7 +871,8 @@ int stc_plan_create(stc_handle h, int nsys, const stc_sys_desc* dev_sys, int mod
     if (items.empty()) return STC_EINVAL;
     const bool refine = mode == STC_REFINED_FP32;
     LaunchCfg cfg;
-    if ((rc = plan_launch(h, sys, (long long)items.size(), refine, 0, -1, &cfg))) return rc;
+    if ((rc = plan_launch(h, sys.data(), (int)sys.size(), (long long)items.size(), refine, 0, 0, &cfg)))
+        return rc;
     stc_plan_s* p = new stc_plan_s();
     p->h = h;
     auto fail = [&](int code) {

@@ -146,10 +146,12 @@ class Handle:
         return X.transpose(0, 2, 1), info, be, st
 
     def sweep_dev(self, rom_id, freq_t, F_t, X_t, info_t, berr_t=None, mode="plain", tol=0.0,
-                  max_iter=0, stream=None):
+                  max_iter=0, stream=None, noalloc=False):
         """Device-resident sweep on torch tensors (no copies, no synchronisation).
         F_t: (nrhs, n) complex128 contiguous (column-major n x nrhs), X_t:
-        (nfreq, nrhs, n) complex128, info_t int32 (nfreq), berr_t float64 or None."""
+        (nfreq, nrhs, n) complex128, info_t int32 (nfreq), berr_t float64 or None.
+        noalloc: run through stc_sweep_dev_noalloc (the library's fixed per-GPU
+        scratch array, no device allocation; calls overlapping on one GPU race)."""
         import torch
         n = self.n[rom_id]
         if F_t.dim() != 2:
@@ -163,13 +165,14 @@ class Handle:
         if berr_t is not None:
             _check_tensor(berr_t, "berr_t", torch.float64, (nf, nrhs), self.device)
         s = _stream_arg(stream)
-        rc = self.lib.stc_sweep_dev(
+        fn = self.lib.stc_sweep_dev_noalloc if noalloc else self.lib.stc_sweep_dev
+        rc = fn(
             self.h, rom_id, freq_t.numel(), ctypes.c_void_p(freq_t.data_ptr()), nrhs,
             ctypes.c_void_p(F_t.data_ptr()), n, ctypes.c_void_p(X_t.data_ptr()), n,
             ctypes.c_void_p(info_t.data_ptr()),
             ctypes.c_void_p(berr_t.data_ptr()) if berr_t is not None else None,
             _mode(mode), float(tol), int(max_iter), s)
-        _check(rc, "stc_sweep_dev")
+        _check(rc, "stc_sweep_dev_noalloc" if noalloc else "stc_sweep_dev")
 
     def getrf(self, rom_id, freq, mode="plain"):
         """LU factors (LAPACK layout, nfreq x n x n), 1-based pivots, info."""
