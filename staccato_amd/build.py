@@ -19,8 +19,10 @@ OUT = os.path.join(OUT_DIR, "libstaccato_sweep.so")
 HOST_SRC = [os.path.join(CSRC, "host", f) for f in ("KrylovROMSweep.cpp", "IJCSACoupling.cpp")]
 HOST_OUT = os.path.join(OUT_DIR, "libstaccato_host.so")
 
-# (refine, rows per lane) pairs of sweep_kernel, one object each
-INSTANCES = [(r, m) for r in (0, 1) for m in (1, 2, 4, 8, 16)]
+# (refine, rows per lane, wide) triples of sweep_kernel, one object each; the
+# list matches STC_INSTANCES in csrc/stc_dispatch.hip
+INSTANCES = ([(r, m, 0) for r in (0, 1) for m in (1, 2, 4, 8, 16)] +
+             [(0, m, 1) for m in (2, 4, 8)] + [(1, m, 1) for m in (2, 4, 8, 16)])
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -49,18 +51,20 @@ def build_library(verbose=False, jobs=None):
     cc = hipcc()
     jobs = jobs or min(16, max(1, os.cpu_count() or 1))
     cmds, objs = [], []
-    for r, m in INSTANCES:
-        o = os.path.join(OBJ_DIR, f"sweep_kernel_{r}_{m}.o")
+    for r, m, wd in INSTANCES:
+        o = os.path.join(OBJ_DIR, f"sweep_kernel_{r}_{m}_{wd}.o")
         cmds.append([cc, *HIPCC_FLAGS, "-c", f"-DSTC_INST_REFINE={r}", f"-DSTC_INST_MAXR={m}",
-                     "-o", o, os.path.join(CSRC, "stc_kern_inst.hip")])
+                     f"-DSTC_INST_WIDE={wd}", "-o", o, os.path.join(CSRC, "stc_kern_inst.hip")])
         objs.append(o)
     for name in ("stc_api", "stc_dispatch"):
         o = os.path.join(OBJ_DIR, name + ".o")
         cmds.append([cc, *HIPCC_FLAGS, "-c", "-o", o, os.path.join(CSRC, name + ".hip")])
         objs.append(o)
     # the longest compilations first
-    order = sorted(range(len(cmds)), key=lambda i: -int(cmds[i][-3].split("=")[-1])
-                   if cmds[i][-3].startswith("-DSTC_INST_MAXR") else 0)
+    def cost(cmd):
+        m = [a for a in cmd if a.startswith("-DSTC_INST_MAXR=")]
+        return int(m[0].split("=")[1]) if m else 0
+    order = sorted(range(len(cmds)), key=lambda i: -cost(cmds[i]))
     with ThreadPoolExecutor(jobs) as ex:
         list(ex.map(_run, [cmds[i] for i in order]))
     tmp = OUT + ".tmp"

@@ -202,9 +202,31 @@ int plan_systems(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, 
     return STC_OK;
 }
 
+// Wide-panel layout of every system within `budget` bytes of LDS; false when
+// one of them does not fit.
+bool layout_wide(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, int budget,
+                 int* lds_out) {
+    int lds = 0;
+    for (int i = 0; i < nsys; ++i) {
+        const Layout L = plan_layout_wide(sys[i].n, sys[i].nrhs, ts, budget, jb_cap, adj, maxr);
+        if (L.lds > budget) return false;
+        lds = std::max(lds, L.lds);
+    }
+    for (int i = 0; i < nsys; ++i) {
+        const Layout L = plan_layout_wide(sys[i].n, sys[i].nrhs, ts, budget, jb_cap, adj, maxr);
+        sys[i].lay_small = L.small;
+        sys[i].lay_jb = L.jb;
+        sys[i].lay_cw = L.cw;
+        sys[i].lay_G = L.G;
+    }
+    *lds_out = lds;
+    return true;
+}
+
 struct LaunchCfg {
     int refine = 0;
     int maxr = 1;
+    int wide = 0;   // sweep_kernel<., ., true>: wide LDS panels on the blocked path
     int lds = 0;
     int grid = 1;
     size_t slot_elems = 0;
@@ -228,7 +250,26 @@ int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine
     const int jb_cap = kWaves * panel_cols_per_wave(ts, maxr);
     int rc = plan_systems(sys, nsys, ts, adj, jb_cap, maxr, &cfg->lds);
     if (rc) return rc;
-    void* kfun = sweep_kernel_fn(refine, maxr);
+    // systems that need a factor slot go to the wide-panel kernel when it is
+    // built for these rows per lane and every system's panel fits its LDS
+    bool blocked = false;
+    for (int i = 0; i < nsys; ++i) blocked = blocked || !sys[i].lay_small;
+    cfg->wide = 0;
+    const char* ew = getenv("STC_WIDE");
+    if (blocked && (!ew || atoi(ew) != 0) && sweep_kernel_fn(refine, maxr, 1)) {
+        int wcap = wide_jb_max(ts);
+        if (const char* e = getenv("STC_JB_MAX")) wcap = std::max(kSB, std::min(atoi(e), wcap));
+        // two workgroups per CU by default: one per CU leaves the pivot and
+        // solve chains of a single workgroup unhidden
+        int budget = kBudgetSmall;
+        if (const char* e = getenv("STC_WIDE_BUDGET")) budget = std::max(16384, std::min(atoi(e), kBudgetLarge));
+        int lds = 0;
+        if (layout_wide(sys, nsys, ts, adj, wcap, maxr, budget, &lds)) {  // writes only when all fit
+            cfg->wide = 1;
+            cfg->lds = lds;
+        }
+    }
+    void* kfun = sweep_kernel_fn(refine, maxr, cfg->wide);
     if (!kfun) return STC_EINVAL;
     HIP_TRY(hipFuncSetAttribute(kfun, hipFuncAttributeMaxDynamicSharedMemorySize, cfg->lds));
     int per_cu = 0;
@@ -241,7 +282,7 @@ int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine
     size_t slot = 0;
     for (int i = 0; i < nsys; ++i)
         if (!sys[i].lay_small)
-            slot = std::max(slot, slot_elems_for(sys[i].n, sys[i].lay_jb));
+            slot = std::max(slot, slot_elems_for(sys[i].n, cfg->wide ? kSB : sys[i].lay_jb));
     cfg->slot_elems = slot;
     cfg->slot_bytes = slot * ts;
     if (slot && slot_budget > 0) {
@@ -265,7 +306,7 @@ int enqueue_raw(const LaunchCfg& cfg, KParams& kp, int* counter, void* scratch, 
     kp.lds_budget = cfg.lds;
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, st, kp.counter);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(sweep_kernel_launch(cfg.refine, cfg.maxr, cfg.grid, cfg.lds, st, kp));
+    HIP_TRY(sweep_kernel_launch(cfg.refine, cfg.maxr, cfg.wide, cfg.grid, cfg.lds, st, kp));
     return STC_OK;
 }
 

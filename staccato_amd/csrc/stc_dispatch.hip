@@ -5,28 +5,34 @@
 
 namespace stc {
 
-#define STC_DECL(R, M)                                                   \
-    void* sweep_kernel_fn_##R##_##M();                                   \
-    hipError_t sweep_kernel_launch_##R##_##M(int, int, hipStream_t, const KParams&);
-STC_DECL(0, 1) STC_DECL(0, 2) STC_DECL(0, 4) STC_DECL(0, 8) STC_DECL(0, 16)
-STC_DECL(1, 1) STC_DECL(1, 2) STC_DECL(1, 4) STC_DECL(1, 8) STC_DECL(1, 16)
+// (refine, rows per lane, wide) triples that are built; the wide-panel kernel
+// is not built where its LDS panel cannot hold 16 columns (complex128 beyond
+// n = 512) or where every system fits the LDS whole (n <= 64).
+#define STC_INSTANCES(X)                                                         \
+    X(0, 1, 0) X(0, 2, 0) X(0, 4, 0) X(0, 8, 0) X(0, 16, 0)                      \
+    X(1, 1, 0) X(1, 2, 0) X(1, 4, 0) X(1, 8, 0) X(1, 16, 0)                      \
+    X(0, 2, 1) X(0, 4, 1) X(0, 8, 1)                                             \
+    X(1, 2, 1) X(1, 4, 1) X(1, 8, 1) X(1, 16, 1)
+
+#define STC_DECL(R, M, W)                                                        \
+    void* sweep_kernel_fn_##R##_##M##_##W();                                     \
+    hipError_t sweep_kernel_launch_##R##_##M##_##W(int, int, hipStream_t, const KParams&);
+STC_INSTANCES(STC_DECL)
 #undef STC_DECL
 
-void* sweep_kernel_fn(int refine, int maxr) {
-#define STC_CASE(R, M) \
-    if (refine == R && maxr == M) return sweep_kernel_fn_##R##_##M();
-    STC_CASE(0, 1) STC_CASE(0, 2) STC_CASE(0, 4) STC_CASE(0, 8) STC_CASE(0, 16)
-    STC_CASE(1, 1) STC_CASE(1, 2) STC_CASE(1, 4) STC_CASE(1, 8) STC_CASE(1, 16)
+void* sweep_kernel_fn(int refine, int maxr, int wide) {
+#define STC_CASE(R, M, W) \
+    if (refine == R && maxr == M && wide == W) return sweep_kernel_fn_##R##_##M##_##W();
+    STC_INSTANCES(STC_CASE)
 #undef STC_CASE
     return nullptr;
 }
 
-hipError_t sweep_kernel_launch(int refine, int maxr, int grid, int lds, hipStream_t st,
+hipError_t sweep_kernel_launch(int refine, int maxr, int wide, int grid, int lds, hipStream_t st,
                                const KParams& kp) {
-#define STC_CASE(R, M) \
-    if (refine == R && maxr == M) return sweep_kernel_launch_##R##_##M(grid, lds, st, kp);
-    STC_CASE(0, 1) STC_CASE(0, 2) STC_CASE(0, 4) STC_CASE(0, 8) STC_CASE(0, 16)
-    STC_CASE(1, 1) STC_CASE(1, 2) STC_CASE(1, 4) STC_CASE(1, 8) STC_CASE(1, 16)
+#define STC_CASE(R, M, W) \
+    if (refine == R && maxr == M && wide == W) return sweep_kernel_launch_##R##_##M##_##W(grid, lds, st, kp);
+    STC_INSTANCES(STC_CASE)
 #undef STC_CASE
     return hipErrorInvalidValue;
 }

@@ -177,4 +177,55 @@ inline Layout plan_layout(int n, int nrhs, int ts, int budget, int jb_cap, int a
     return L;
 }
 
+// ---- wide-panel blocked path (sweep_kernel<T, MAXR, true>) ----------------------------
+constexpr int kSB = 16;  // sub-panel width and inverted diagonal block size
+
+// Widest panel the wide kernel keeps in registers for the trailing update:
+// 64 columns for complex64, 32 for complex128.
+__host__ __device__ constexpr int wide_jb_max(int elem_bytes) { return elem_bytes == 8 ? 64 : 32; }
+
+// LDS of the wide path beyond lds_fixed_bytes: the n x jb panel, its inverted
+// 16 x 16 diagonal blocks, the sub-panel's two alternating pivot-candidate
+// buffers (per wave one row of 16 and its 16-byte key) and the moved-row
+// list (2 x 16).  maxr is not needed (kept for the callers' signature).
+__host__ __device__ inline long long wide_cand_bytes(int ts) { return 2LL * kWaves * kSB * ts + 2LL * kWaves * 16; }
+__host__ __device__ inline long long wide_lds_bytes(int n, int jb, int ts, int maxr) {
+    (void)maxr;
+    return align16((long long)n * jb * ts) + align16((long long)jb * kSB * ts) + align16(wide_cand_bytes(ts)) +
+           align16(2LL * kSB * 8);
+}
+
+// Layout of a system in a wide-panel launch: the small path when the matrix
+// fits the LDS budget whole, else the widest panel (a multiple of 16 up to
+// jb_cap) that fits.
+inline Layout plan_layout_wide(int n, int nrhs, int ts, int budget, int jb_cap, int adj, int maxr) {
+    Layout L{0, 0, 0, 1, 0};
+    const int fixed = lds_fixed_bytes(n);
+    int Gmax = adj ? 1 : (nrhs < kMaxGroup ? nrhs : kMaxGroup);
+    if (Gmax < 1) Gmax = 1;
+    for (int G = Gmax; G >= 1; --G) {
+        int need = fixed + align16((long long)n * n * ts) + lds_solve_bytes(n, G, ts, adj);
+        if (need <= budget) {
+            L.small = 1; L.G = G; L.lds = need;
+            return L;
+        }
+    }
+    for (int jb = wide_jb_max(ts); jb >= kSB; jb -= kSB) {
+        if (jb > jb_cap) continue;
+        const long long need = fixed + wide_lds_bytes(n, jb, ts, maxr);
+        if (need <= budget) {
+            L.jb = jb; L.cw = kSB; L.lds = (int)need;
+            for (int G = Gmax; G >= 1; --G) {
+                int ns = fixed + lds_solve_bytes(n, G, ts, adj) +
+                         align16((long long)kSolveBlk * kSolveBlk * ts);
+                if (ns <= budget) { L.G = G; if (ns > L.lds) L.lds = ns; return L; }
+            }
+            L.lds = 0x7fffffff;
+            return L;
+        }
+    }
+    L.lds = 0x7fffffff;
+    return L;
+}
+
 }  // namespace stc

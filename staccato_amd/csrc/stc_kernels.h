@@ -75,6 +75,18 @@ __device__ __forceinline__ T lds1(const T* p) {
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 template <typename T>
+__device__ __forceinline__ cx<T> cmk(T r, T i);
+// Load that every lane issues, its address clamped into range by the caller,
+// zeroed afterwards where the element does not exist.  A load under a
+// divergent branch makes the compiler wait for it before the branch joins;
+// these issue as one batch and are waited for once.
+template <int AS, typename T>
+__device__ __forceinline__ cx<T> ldz(const cx<T>* p, bool ok) {
+    const cx<T> v = ldc<AS>(p);
+    return cmk<T>(ok ? v.re : T(0), ok ? v.im : T(0));
+}
+
+template <typename T>
 __device__ __forceinline__ cx<T> cmk(T r, T i) {
     cx<T> z;
     z.re = r;
@@ -159,7 +171,12 @@ __device__ __forceinline__ z64 ld_z(const double2* p) {
 __device__ __forceinline__ void st_z(double2* p, z64 a) { *p = make_double2(a.re, a.im); }
 // NaN-propagating max
 __device__ __forceinline__ double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
-__device__ __forceinline__ double zabs(z64 a) { return hypot(a.re, a.im); }
+// |a| for norms: sqrt(re^2 + im^2), falling back to hypot's scaling only
+// when the squares overflow or underflow
+__device__ __forceinline__ double zabs(z64 a) {
+    const double q = fma(a.re, a.re, a.im * a.im);
+    return (q < 1e300 && q > 1e-300) || q == 0.0 ? sqrt(q) : hypot(a.re, a.im);
+}
 
 // ---- assembly: element e (column-major, ld = n) of A(w) in FP64 ---------------
 __device__ __forceinline__ z64 assemble_at(const SysDesc& s, size_t e, double w, double w2) {
@@ -928,63 +945,6 @@ __device__ __forceinline__ void rect_update(const TriCtx<T>& t, int i, int k0, i
     else rect_update_as<0, T, CONJ, TRANS>(t, i, k0, kb, w, gc);
 }
 
-// y_b = Binv_b w_b for one diagonal block (lanes = rows, one wave per column);
-// Binv_b in the global slot.
-template <typename T, bool LOWER>
-__device__ __forceinline__ void block_inv_apply(const cx<T>* __restrict__ Bi, int k0, int kb, cx<T>* w,
-                                                int n, int gc) {
-    constexpr int KR = SolveRound<T>::K;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    k0 = __builtin_amdgcn_readfirstlane(k0);
-    kb = __builtin_amdgcn_readfirstlane(kb);
-    gc = __builtin_amdgcn_readfirstlane(gc);
-    n = __builtin_amdgcn_readfirstlane(n);
-    const int row = min(lane, kb - 1);
-    for (int g = wave; g < gc; g += kWaves) {
-        cx<T>* y = w + (size_t)g * n + k0;
-        cx<T> acc = czero<T>();
-        for (int k8 = 0; k8 < kb; k8 += KR) {
-            cx<T> bv[KR];
-#pragma unroll
-            for (int u = 0; u < KR; ++u) bv[u] = ldc<1>(Bi + row + (size_t)min(k8 + u, kb - 1) * kb);
-#pragma unroll
-            for (int u = 0; u < KR; ++u) {
-                const int k = k8 + u;
-                const bool in = k < kb && (LOWER ? k <= lane : k >= lane);
-                if (k < kb) cfma(acc, in ? bv[u] : czero<T>(), ldc<3>(y + k));
-            }
-        }
-        if (lane < kb) stc<3>(y + lane, acc);
-    }
-}
-
-// L unit lower, forward, with the inverted diagonal blocks.
-template <typename T>
-__device__ __forceinline__ void lower_unit_solve_inv(const TriCtx<T>& t, cx<T>* w, int gc) {
-    const int tid = threadIdx.x, n = t.n;
-    for (int k0 = 0, b = 0; k0 < n; k0 += t.bs, ++b) {
-        const int kb = min(t.bs, n - k0);
-        block_inv_apply<T, true>(t.Linv + (size_t)b * t.bs * t.bs, k0, kb, w, n, gc);
-        __syncthreads();
-        for (int i = k0 + kb + tid; i < n; i += kThreads) rect_update<T, false, false>(t, i, k0, kb, w, gc);
-        __syncthreads();
-    }
-}
-
-// U upper, backward, with the inverted diagonal blocks.
-template <typename T>
-__device__ __forceinline__ void upper_solve_inv(const TriCtx<T>& t, cx<T>* w, int gc) {
-    const int tid = threadIdx.x, n = t.n;
-    const int nb = (n + t.bs - 1) / t.bs;
-    for (int b = nb - 1; b >= 0; --b) {
-        const int k0 = b * t.bs, kb = min(t.bs, n - k0);
-        block_inv_apply<T, false>(t.Uinv + (size_t)b * t.bs * t.bs, k0, kb, w, n, gc);
-        __syncthreads();
-        for (int i = tid; i < k0; i += kThreads) rect_update<T, false, false>(t, i, k0, kb, w, gc);
-        __syncthreads();
-    }
-}
-
 // L unit lower, forward.
 template <typename T>
 __device__ __forceinline__ void lower_unit_solve(const TriCtx<T>& t, cx<T>* w, int gc) {
@@ -1083,6 +1043,146 @@ __device__ __forceinline__ void lower_h_unit_solve(const TriCtx<T>& t, cx<T>* w,
         for (int i = tid; i < k0; i += kThreads) rect_update<T, true, true>(t, i, k0, kb, w, gc);
         __syncthreads();
     }
+}
+
+// ---- pipelined blocked triangular solves (factors in the global slot) ----------------------
+// Solves op(LU) on the LDS vectors w (n x gc, ld n) in 16-row blocks.  No
+// global load of a block depends on the solution (neither its diagonal block
+// or stored inverse nor the rows of its off-diagonal update), so block b+1's
+// loads are issued before block b is solved and the factor slot's latency
+// overlaps the LDS work.  Per block: the diagonal block (or its inverse) is
+// staged in LDS from one prefetched element per thread, then solved (INV:
+// y = Binv y, one lane per (row, column); else substitution, one wave per
+// column, unit diagonal when UNIT), then every thread updates its row of the
+// rows outside the block from its prefetched 16 entries (further rows of
+// n > 256 load theirs on demand).  FWD: blocks in increasing order (lower
+// triangular op), else decreasing.  TRANS: op(LU)(i, k) = conj(LU(k, i)).
+template <typename T, bool FWD, bool INV, bool UNIT, bool TRANS>
+__device__ __forceinline__ void tri_solve_pipe(const TriCtx<T>& t, const cx<T>* __restrict__ binv, cx<T>* w,
+                                               int gc) {
+    constexpr int B = 16;
+    constexpr bool CONJ = TRANS;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = __builtin_amdgcn_readfirstlane(t.n);
+    gc = __builtin_amdgcn_readfirstlane(gc);
+    const int ld = __builtin_amdgcn_readfirstlane(t.ld);
+    const int bs = INV ? __builtin_amdgcn_readfirstlane(t.bs) : B;
+    const int nb = (n + bs - 1) / bs;
+    const cx<T>* __restrict__ LU = t.LU;
+    cx<T>* Dg = t.Dg;
+    const int i0 = tid;  // this thread's prefetched row
+    const int i0c = min(i0, n - 1);
+    const size_t pr0 = (size_t)lds1<3>(t.rmap + i0c);
+    auto in_rows = [&](int i, int k0, int kb) { return FWD ? (i >= k0 + kb && i < n) : (i < k0); };
+    auto load_diag = [&](int b) -> cx<T> {
+        const int k0 = b * bs, kb = min(bs, n - k0);
+        const int i = tid & (B - 1), j = tid >> 4;
+        const int ic = min(i, kb - 1), jc = min(j, kb - 1);
+        const bool ok = i < kb && j < kb;
+        (void)ok;  // entries outside the kb x kb block are staged but never read
+        if (INV) return ldc<1>(binv + ((size_t)b * bs * bs + ic + (size_t)jc * kb));
+        return ldc<1>(LU + ((size_t)lds1<3>(t.rmap + k0 + ic) + (size_t)(k0 + jc) * ld));
+    };
+    auto load_rect = [&](int b, cx<T> (&r)[B]) {
+        const int k0 = b * bs, kb = min(bs, n - k0);
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int k = k0 + min(u, kb - 1);
+            // rows outside the update and entries beyond kb are loaded (clamped) but never used
+            r[u] = TRANS ? ldc<1>(LU + ((size_t)lds1<3>(t.rmap + k) + (size_t)i0c * ld))
+                         : ldc<1>(LU + (pr0 + (size_t)k * ld));
+        }
+    };
+    cx<T> dc, dn = czero<T>();
+    cx<T> rc[B], rn[B];
+    {
+        const int b = FWD ? 0 : nb - 1;
+        dc = load_diag(b);
+        load_rect(b, rc);
+    }
+    for (int st = 0; st < nb; ++st) {
+        const int b = FWD ? st : nb - 1 - st;
+        const int k0 = b * bs, kb = min(bs, n - k0);
+        stc<3>(Dg + tid, dc);  // element (tid % 16, tid / 16), ld 16
+        if (st + 1 < nb) {
+            const int bn = FWD ? b + 1 : b - 1;
+            dn = load_diag(bn);
+            load_rect(bn, rn);
+        }
+        __syncthreads();
+        if (INV) {
+            if (tid < B * gc) {
+                const int r = tid & (B - 1), g = tid >> 4;
+                cx<T>* y = w + (size_t)g * n + k0;
+                cx<T> acc = czero<T>();
+#pragma unroll
+                for (int k = 0; k < B; ++k)
+                    if (k < kb) cfma(acc, ldc<3>(Dg + (r + k * B)), ldc<3>(y + k));
+                if (r < kb) stc<3>(y + r, acc);  // one wave: every read above precedes the writes
+            }
+        } else {
+            for (int g = wave; g < gc; g += kWaves) {
+                cx<T>* y = w + (size_t)g * n + k0;
+                cx<T> v = lane < kb ? ldc<3>(y + lane) : czero<T>();
+                if (FWD) {
+                    for (int k = 0; k < kb; ++k) {
+                        if (!UNIT && lane == k) {
+                            const cx<T> d = ldc<3>(Dg + (k + k * B));
+                            v = cdiv(v, CONJ ? cconj(d) : d);
+                        }
+                        const cx<T> vk = cbcast(v, k);
+                        if (lane > k && lane < kb) {
+                            const cx<T> a = TRANS ? ldc<3>(Dg + (k + lane * B)) : ldc<3>(Dg + (lane + k * B));
+                            cfms(v, CONJ ? cconj(a) : a, vk);
+                        }
+                    }
+                } else {
+                    for (int k = kb - 1; k >= 0; --k) {
+                        if (!UNIT && lane == k) {
+                            const cx<T> d = ldc<3>(Dg + (k + k * B));
+                            v = cdiv(v, CONJ ? cconj(d) : d);
+                        }
+                        const cx<T> vk = cbcast(v, k);
+                        if (lane < k) {
+                            const cx<T> a = TRANS ? ldc<3>(Dg + (k + lane * B)) : ldc<3>(Dg + (lane + k * B));
+                            cfms(v, CONJ ? cconj(a) : a, vk);
+                        }
+                    }
+                }
+                if (lane < kb) stc<3>(y + lane, v);
+            }
+        }
+        __syncthreads();
+        // off-diagonal rows: w[i] -= op(LU)(i, k0:k0+kb) w[k0:k0+kb]
+        if (in_rows(i0, k0, kb)) {
+            cx<T> acc[kMaxGroup];
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                if (u < kb) {
+                    const cx<T> a = CONJ ? cconj(rc[u]) : rc[u];
+#pragma unroll
+                    for (int g = 0; g < kMaxGroup; ++g)
+                        if (g < gc) cfma(acc[g], a, ldc<3>(w + (k0 + u + (size_t)g * n)));
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < kMaxGroup; ++g)
+                if (g < gc) {
+                    cx<T> a = ldc<3>(w + (i0 + (size_t)g * n));
+                    a.re -= acc[g].re;
+                    a.im -= acc[g].im;
+                    stc<3>(w + (i0 + (size_t)g * n), a);
+                }
+        }
+        for (int i = i0 + kThreads; i < n; i += kThreads)
+            if (in_rows(i, k0, kb)) rect_update_as<1, T, CONJ, TRANS>(t, i, k0, kb, w, gc);
+        dc = dn;
+#pragma unroll
+        for (int u = 0; u < B; ++u) rc[u] = rn[u];
+    }
+    __syncthreads();
 }
 
 // ---- block-wide max of the kRedSlots values (NaN propagating) --------------------------
@@ -1254,12 +1354,20 @@ __device__ __forceinline__ int solve_group(const SysDesc& s, double w, double w2
     // fwd_done: wv already holds L^-1 P b (eliminated with the factorisation),
     // so the first solve is the U solve only
     auto solve_t = [&]() {
-        if (!ADJ) {
+        if (tri.rmap) {  // factors in the global slot: pipelined block solves
+            if (!ADJ) {
+                if (fwd_done) fwd_done = false;
+                else tri_solve_pipe<T, true, true, true, false>(tri, tri.Linv, wv, gc);
+                if (tri.inv_u) tri_solve_pipe<T, false, true, false, false>(tri, tri.Uinv, wv, gc);
+                else tri_solve_pipe<T, false, false, false, false>(tri, nullptr, wv, gc);
+            } else {
+                tri_solve_pipe<T, true, false, false, true>(tri, nullptr, wv, gc);
+                tri_solve_pipe<T, false, false, true, true>(tri, nullptr, wv, gc);
+            }
+        } else if (!ADJ) {
             if (fwd_done) fwd_done = false;
-            else if (tri.Linv) lower_unit_solve_inv(tri, wv, gc);
             else lower_unit_solve(tri, wv, gc);
-            if (tri.Linv && tri.inv_u) upper_solve_inv(tri, wv, gc);
-            else upper_solve(tri, wv, gc);
+            upper_solve(tri, wv, gc);
         } else {
             upper_h_solve(tri, wv, gc);
             lower_h_unit_solve(tri, wv, gc);
@@ -1420,8 +1528,17 @@ __device__ __forceinline__ void adjoint_point(const SysDesc& s, int fidx, double
     __syncthreads();
 }
 
+}  // namespace stc
+
+#include "stc_wide.h"
+
+namespace stc {
+
 // ---- one point ---------------------------------------------------------------------------
-template <typename T, int MAXR>
+// WIDE: blocked systems use the wide-panel factorisation (factor_wide, LDS
+// panel of lay.jb columns, 16 x 16 inverted diagonal blocks); otherwise the
+// register panel (factor_blocked, lay.jb x lay.jb blocks).
+template <typename T, int MAXR, bool WIDE>
 __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const KParams& kp, char* smem) {
     const int n = s.n, tid = threadIdx.x, nrhs = s.nrhs;
     const double w = s.freq ? kTwoPi * s.freq[fidx] : 0.0, w2 = w * w;
@@ -1449,8 +1566,9 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
         LU = A;
         solvep = L.main + align16((long long)nn * sizeof(cx<T>));
     } else {
-        if (kp.scratch == nullptr || kp.slot_elems < slot_elems_for(n, lay.jb) ||
-            lay.jb > kWaves * panel_cols_per_wave((int)sizeof(cx<T>), MAXR) || lay.jb > 32) {
+        const bool bad_jb = WIDE ? (lay.jb < kSB || lay.jb % kSB != 0 || lay.jb > kSB * WideCfg<T>::NB)
+                                 : (lay.jb > kWaves * panel_cols_per_wave((int)sizeof(cx<T>), MAXR) || lay.jb > 32);
+        if (kp.scratch == nullptr || kp.slot_elems < slot_elems_for(n, WIDE ? kSB : lay.jb) || bad_jb) {
             // no factor slot, or a panel wider than the register panel (the host
             // plans neither): report, write NaN, touch nothing else
             double2* Xn = s.X ? s.X + (size_t)fidx * s.ldx * nrhs : nullptr;
@@ -1467,7 +1585,8 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
         // right-hand sides eliminated with the factorisation (the first group of
         // a forward sweep)
         const int nr = (kp.flags & (kAdjoint | kFactorOut)) ? 0 : min(nrhs, lay.G);
-        factor_blocked<T, MAXR>(s, w, w2, S, lay, L, nr);
+        if (WIDE) factor_wide<T, MAXR>(s, w, w2, S, lay, L, nr);
+        else factor_blocked<T, MAXR>(s, w, w2, S, lay, L, nr);
         fused = nr;
         LU = S;
         solvep = L.main;
@@ -1520,10 +1639,10 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
     tri.n = n;
     tri.rmap = lay.small ? nullptr : L.rmap;
     tri.Dg = lay.small ? nullptr : reinterpret_cast<cx<T>*>(after);
-    tri.bs = lay.jb;
+    const int bs = WIDE ? kSB : lay.jb;  // inverted diagonal blocks of the slot
+    tri.bs = bs;
     tri.Linv = lay.small ? nullptr : LU + (size_t)n * (n + kMaxGroup);
-    tri.Uinv = lay.small ? nullptr
-                         : LU + (size_t)n * (n + kMaxGroup) + (size_t)((n + lay.jb - 1) / lay.jb) * lay.jb * lay.jb;
+    tri.Uinv = lay.small ? nullptr : LU + (size_t)n * (n + kMaxGroup) + (size_t)((n + bs - 1) / bs) * bs * bs;
     tri.inv_u = refine ? 1 : 0;
     if (kp.flags & kAdjoint) {
         z64* xk = reinterpret_cast<z64*>(reinterpret_cast<char*>(rs) + align16((long long)n * G * 16));
@@ -1586,10 +1705,11 @@ __device__ __forceinline__ SysDesc load_uniform(const SysDesc* p) {
 // ---- the persistent kernel -------------------------------------------------------------
 // wg_per_cu(sizeof(cx<T>)) workgroups of 4 wavefronts per CU: the register
 // file is split that many ways (at most 168 registers per wavefront for three,
-// 256 for two).
-template <typename T, int MAXR>
+// 256 for two).  The wide-panel kernel (WIDE) runs one or two per CU (its LDS
+// panel decides) with up to 256 registers.
+template <typename T, int MAXR, bool WIDE>
 __global__ void __launch_bounds__(kThreads)
-    __attribute__((amdgpu_waves_per_eu(wg_per_cu(sizeof(cx<T>)), wg_per_cu(sizeof(cx<T>)))))
+    __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : wg_per_cu(sizeof(cx<T>)), WIDE ? 2 : wg_per_cu(sizeof(cx<T>)))))
 sweep_kernel(KParams kp) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ int s_item;
@@ -1615,7 +1735,7 @@ sweep_kernel(KParams kp) {
             if (s.has_D) s.D += o;
         }
         if (s.f_stride) s.F += (long long)fidx * s.f_stride;
-        process_point<T, MAXR>(s, fidx, kp, smem);
+        process_point<T, MAXR, WIDE>(s, fidx, kp, smem);
         __syncthreads();
     }
 }
