@@ -131,15 +131,35 @@ def synth_closed_last_chunk(case, chunk):
     return synth.closed_form(case.rom, case.freq[NFREQ - chunk:], case.F)
 
 
+def host_cpu_info():
+    """Logical CPUs of the host, those this process may run on, and the CPU
+    share it uses (OMP_NUM_THREADS when set: the share a GPU job is given)."""
+    total = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = total
+    share = avail
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        share = max(1, min(avail, int(os.environ["OMP_NUM_THREADS"])))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return total, avail, share, model
+
+
 def cpu_baseline(case, budget_s=20.0):
-    """Reference loop on the host cores (OpenBLAS zgetrf/zgetrs per frequency)."""
+    """Reference loop on the host cores (OpenBLAS zgetrf/zgetrs per frequency),
+    OpenMP over frequencies with the process's CPU share, plus one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    total, avail, cores, model = host_cpu_info()
     rom = case.rom
     lib = oracle.openblas_path()
     # one full 4096-frequency sweep calibrates; whole sweeps then repeat for ~budget_s
@@ -157,9 +177,26 @@ def cpu_baseline(case, budget_s=20.0):
     t = 0.0
     for _ in range(reps):
         t += fn(case.freq)
+    # one thread: a quarter sweep (every 4th frequency)
+    if lib is not None and "OpenBLAS" in kind_lib:
+        fn1 = lambda fr: oracle.sweep_lapack(rom.K, rom.M, rom.D, rom.eta, fr, case.F, nthreads=1)[2]
+    else:
+        fn1 = lambda fr: oracle.sweep(rom.K, rom.M, rom.D, rom.eta, fr, case.F, nthreads=1,
+                                      berr=False)[3]
+    fr1 = case.freq[::4]
+    t1 = fn1(fr1)
     return {"value": reps * NFREQ / t, "unit": "freq points/s", "cores": cores, "kind": "port",
             "sample": f"{reps} x the full {NFREQ}-frequency sweep (n=256, 4 RHS), {kind_lib}, "
-                      f"OpenMP over frequencies, 1 LAPACK thread each, {t:.1f} s"}
+                      f"OpenMP over frequencies, {cores} threads (the process's CPU share; the host "
+                      f"has {total} logical CPUs, {avail} available: {model}), 1 LAPACK thread each, "
+                      f"{t:.1f} s",
+            "one_thread": {"value": fr1.size / t1, "unit": "freq points/s", "cores": 1,
+                           "sample": f"{fr1.size} frequencies (every 4th), {t1:.1f} s"},
+            "host_logical_cpus": total, "available_cpus": avail}
+
+
+def pts_total(cases):
+    return sum(c.freq.size for c in cases)
 
 
 def other_workload(name, steps, warmup):
@@ -190,14 +227,39 @@ def other_workload(name, steps, warmup):
             d.F, d.ldf, d.X, d.ldx = t[1].data_ptr(), n, t[2].data_ptr(), n
             d.info, d.berr = t[3].data_ptr(), t[4].data_ptr()
             W += nf * work_per_point(n, nr)
-        plan = Plan(h, descs, mode="refined", tol=TOL_REFINED)
+        # launch policy by measurement: the plan's launch is keyed by its largest order
+        from staccato_amd import autotune
+        nmax = max(c.rom.n for c in cases)
         stream = torch.cuda.current_stream()
+        trial = {}
+        for name, kernel, budget, jb in autotune.CANDIDATES:
+            h.set_policy(nmax, "refined", kernel, budget, jb)
+            p = Plan(h, descs, mode="refined", tol=TOL_REFINED)
+            w_, _ = time_gpu(lambda: p.execute(), 2, 1, stream)
+            ok = all(int((t[3] != 0).sum()) == 0 for t in keep)
+            trial[name] = (pts_total(cases) * 2 / w_ if ok else 0.0, kernel, budget, jb)
+            p.close()
+        best = max(trial, key=lambda k: trial[k][0])
+        h.set_policy(nmax, "refined", *trial[best][1:])
+        out["autotune"] = {"chosen": best, "candidates_pts_per_s": {k: round(v[0]) for k, v in trial.items()}}
+        plan = Plan(h, descs, mode="refined", tol=TOL_REFINED)
         wall, _ = time_gpu(lambda: plan.execute(), steps, warmup, stream)
         pts = sum(c.freq.size for c in cases)
         bmax = max(float(t[4].max()) for t in keep)
         bad = sum(int((t[3] != 0).sum()) for t in keep)
+        # per-system error against the FP64 closed-form solution (every 16th frequency)
+        errs = []
+        for c, t in zip(cases, keep):
+            sel = np.arange(0, c.freq.size, 16)
+            Xh = t[2].cpu().numpy().transpose(0, 2, 1)[sel]
+            Xc = synth.closed_form(c.rom, c.freq[sel], c.F)
+            errs.append(float((np.abs(Xh - Xc).max(axis=(1, 2)) / np.abs(Xc).max(axis=(1, 2))).max()))
+        worst = int(np.argmax(errs))
         out.update(points=pts, pts_per_s=pts * steps / wall, tflops=W * steps / wall / 1e12,
-                   max_backward_error=bmax, points_failed=bad)
+                   max_backward_error=bmax, points_failed=bad,
+                   per_system_max_rel_error_vs_fp64_closed_form=errs,
+                   worst_system={"index": worst, "n": int(cases[worst].rom.n),
+                                 "rel_error": errs[worst]})
         plan.close()
     elif name == "adjoint":
         c = synth.adjoint_n128()
@@ -225,13 +287,31 @@ def other_workload(name, steps, warmup):
         info = torch.empty(c.freq.size, dtype=torch.int32, device=dev)
         be = torch.empty((c.freq.size, 16), dtype=torch.float64, device=dev)
         stream = torch.cuda.Stream(device=dev)
+        from staccato_amd import autotune
+        out["autotune"] = {}
         for mode in ("refined", "plain"):
+            tol = TOL_REFINED if mode == "refined" else TOL_PLAIN
+            best, cand = autotune.tune(h, 0, c.freq, c.F, mode, tol, sample=c.freq.size, reps=1,
+                                       cache=False)
+            out["autotune"][mode] = {"chosen": best["name"] if best else "automatic",
+                                     "candidates_pts_per_s": {q["name"]: round(q["pts_per_s"]) for q in cand}}
             run = lambda: h.sweep_dev(0, fr, F, X, info, be if mode == "refined" else None,
                                       mode=mode, tol=TOL_REFINED, max_iter=10,
                                       stream=stream.cuda_stream)
-            wall, _ = time_gpu(run, steps, warmup, stream)
-            out[mode] = {"pts_per_s": c.freq.size * steps / wall,
-                         "tflops": c.freq.size * work_per_point(1024, 16) * steps / wall / 1e12}
+            wall, ev = time_gpu(run, steps, warmup, stream)
+            h.sweep_dev(0, fr, F, X, info, be, mode=mode, tol=tol, max_iter=10, stream=stream.cuda_stream)
+            torch.cuda.synchronize()
+            Xh = X.cpu().numpy().transpose(0, 2, 1)
+            sel = np.arange(0, c.freq.size, 8)  # closed form on every 8th frequency (host FP64)
+            Xc = synth.closed_form(rom, c.freq[sel], c.F)
+            fe = np.abs(Xh[sel] - Xc).max(axis=(1, 2)) / np.abs(Xc).max(axis=(1, 2))
+            tfl = c.freq.size * work_per_point(1024, 16) * steps / ev / 1e12
+            peak = PEAK_FP32_TFLOPS if mode == "refined" else PEAK_FP64_TFLOPS
+            out[mode] = {"pts_per_s": c.freq.size * steps / wall, "tflops": tfl,
+                         "mfma_peak_frac": tfl / peak,
+                         "max_backward_error": float(be.max()),
+                         "points_failed": int((info != 0).sum()),
+                         "max_rel_error_vs_closed_form": float(fe.max())}
     elif name == "ijcsa":
         # interface-Jacobian coupling: a chain of three n=256 substructures
         # (16 shared DoFs per joint, 4 loads, 8 outputs each), 4096 frequencies
@@ -324,7 +404,7 @@ def main():
         return
 
     import torch
-    from staccato_amd import Handle, synth
+    from staccato_amd import Handle, autotune, synth
 
     case = synth.uniform_n256(NFREQ)
     dev = torch.device("cuda:0")
@@ -366,6 +446,16 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
+    # launch policy per mode picked by measurement (staccato_amd.autotune): every
+    # candidate timed on the full sweep, the fastest whose points all meet tol
+    tuned = {}
+    for mode, tol in (("refined", TOL_REFINED), ("plain", TOL_PLAIN)):
+        if args.mode in ("both", mode):
+            best, cand = autotune.tune(h, 0, case.freq, case.F, mode, tol, sample=NFREQ, reps=2,
+                                       cache=False)
+            tuned[mode] = {"chosen": best["name"] if best else "automatic",
+                           "candidates_pts_per_s": {c["name"]: round(c["pts_per_s"]) for c in cand}}
+
     res = {}
     for mode, tol in (("refined", TOL_REFINED), ("plain", TOL_PLAIN)):
         if args.mode not in ("both", mode):
@@ -406,6 +496,7 @@ def main():
                      "traffic_unit": "bytes per launch (HBM, rocprof FETCH_SIZE/WRITE_SIZE)",
                      "work_per_point_flop": W, "points_per_launch": NFREQ},
         "results": res,
+        "autotune": tuned,
     }
     if "plain" in res and "refined" in res:
         out["speedup_vs_plain_fp64"] = res["refined"]["pts_per_s"] / res["plain"]["pts_per_s"]

@@ -104,6 +104,26 @@ const char* stc_strerror(int code);
 /* 1 when the library was built for and runs on gfx950, else 0 (no GPU). */
 int stc_device_ok(stc_handle h);
 
+/* ---- launch policy (set by an autotuner; optional) -----------------------------
+ * How systems that do not fit the LDS whole are factored.  kernel: 0 =
+ * automatic, 1 = 16-column register panel (three workgroups per CU for
+ * complex64, two for complex128), 2 = wide panel held in LDS (16..64 columns,
+ * one or two workgroups per CU, by lds_budget).  lds_budget: bytes of LDS per
+ * workgroup, 0 = automatic, else 16384 .. 163584 (the LDS the panel may use
+ * sets the workgroups per CU and the panel width).  jb_max: widest panel
+ * (0 = no cap).  An entry applies to launches whose largest system has order
+ * n in that mode, or the nearest order with an entry; NULL removes it.  The
+ * results do not depend on the policy beyond rounding (pivots follow LAPACK
+ * in every kernel).  No counterpart in the reference (its only tuning knob is a
+ * fixed batch size, test/krylovMidsizeTest/gpu/concurrent/batchedLU/src/config/config.cu:29-30). */
+typedef struct {
+    int kernel;
+    int lds_budget;
+    int jb_max;
+} stc_policy;
+int stc_set_policy(stc_handle h, int n, int mode, const stc_policy* p);
+int stc_clear_policies(stc_handle h);
+
 /* ---- ROM operators ------------------------------------------------------------
  * Copies K, M, D (n x n, leading dimension ld, host memory) to the device.
  * D may be NULL (undamped).  eta is the structural loss factor applied to K;

@@ -8,5 +8,6 @@ is include/staccato_sweep.h; this package is its Python host side.
 from .sweep import Handle, Plan, StcError, MODES  # noqa: F401
 from . import synth  # noqa: F401
 from . import coupling  # noqa: F401
+from . import autotune  # noqa: F401
 
-__all__ = ["Handle", "Plan", "StcError", "MODES", "synth", "coupling"]
+__all__ = ["Handle", "Plan", "StcError", "MODES", "synth", "coupling", "autotune"]

@@ -25,7 +25,7 @@ STC_REFINED_FP32 = 1
 
 # every symbol include/staccato_sweep.h declares
 EXPORTED = (
-    "stc_create", "stc_destroy", "stc_strerror", "stc_device_ok",
+    "stc_create", "stc_destroy", "stc_strerror", "stc_device_ok", "stc_set_policy", "stc_clear_policies",
     "stc_rom_set", "stc_rom_set_io", "stc_rom_clear",
     "stc_sweep", "stc_sweep_dev", "stc_sweep_dev_noalloc", "stc_getrf",
     "stc_sweep_batch", "stc_plan_create", "stc_plan_execute", "stc_plan_destroy",
@@ -41,6 +41,14 @@ class Stats(ctypes.Structure):
         ("n_singular", ctypes.c_int),
         ("n_not_converged", ctypes.c_int),
         ("max_backward_error", ctypes.c_double),
+    ]
+
+
+class Policy(ctypes.Structure):
+    _fields_ = [
+        ("kernel", ctypes.c_int),
+        ("lds_budget", ctypes.c_int),
+        ("jb_max", ctypes.c_int),
     ]
 
 
@@ -88,6 +96,8 @@ def load(path=None):
         "stc_destroy": (i, [vp]),
         "stc_strerror": (ctypes.c_char_p, [i]),
         "stc_device_ok": (i, [vp]),
+        "stc_set_policy": (i, [vp, i, i, vp]),
+        "stc_clear_policies": (i, [vp]),
         "stc_rom_set": (i, [vp, i, i, vp, vp, vp, i, d]),
         "stc_rom_set_io": (i, [vp, i, i, vp, i, i, vp, i]),
         "stc_rom_clear": (i, [vp, i]),

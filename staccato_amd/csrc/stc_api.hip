@@ -84,6 +84,15 @@ struct LaunchOrder {
 
 }  // namespace
 
+// How the blocked path of a launch runs (stc_policy): kernel 0 = automatic,
+// 1 = 16-column register panel, 2 = wide LDS panel; lds_budget bytes per
+// workgroup (0 = automatic); jb_max caps the panel width (0 = none).
+struct Policy {
+    int kernel = 0;
+    int lds_budget = 0;
+    int jb_max = 0;
+};
+
 struct stc_ctx {
     int device = 0;
     int num_cu = 0;
@@ -94,6 +103,7 @@ struct stc_ctx {
     DevBuf scratch, counter;
     LaunchOrder order;
     int live_plans = 0;
+    std::map<std::pair<int, int>, Policy> policy;  // (order n, refine) -> policy
     // staging for host-memory calls
     DevBuf freq, F, X, info, berr, iters, fac, ipiv, sys, items, adj, c, phi, kd, A;
 };
@@ -162,15 +172,16 @@ constexpr int kBudgetThird = 53 * 1024;       // three workgroups per CU
 constexpr int kBudgetSmall = 80 * 1024;       // two workgroups per CU
 constexpr int kBudgetLarge = 160 * 1024 - 256;  // one workgroup per CU
 
-// Pick the LDS budget of a launch and fill each system's layout: the largest
-// number of workgroups per CU the kernel is built for at which every blocked
-// system still gets a panel of at least 12 columns (a thin panel starves the
+// Pick the LDS budget of a launch (forced_budget > 0: that one) and fill each
+// system's layout for the register-panel kernel: the largest number of
+// workgroups per CU the kernel is built for at which every blocked system
+// still gets a panel of at least 12 columns (a thin panel starves the
 // trailing update).
-int plan_systems(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, int* lds_out) {
+int plan_systems(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, int forced_budget,
+                 int* lds_out) {
     int budget = kBudgetLarge;
-    if (const char* e = getenv("STC_JB_MAX")) jb_cap = std::max(4, std::min(atoi(e), jb_cap));
-    if (const char* e = getenv("STC_LDS_BUDGET")) {
-        budget = std::max(16384, std::min(atoi(e), kBudgetLarge));
+    if (forced_budget > 0) {
+        budget = forced_budget;
     } else {
         const int cands[] = {kBudgetThird, kBudgetSmall};
         for (int b : cands) {
@@ -223,6 +234,22 @@ bool layout_wide(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, 
     return true;
 }
 
+// Policy for a launch whose largest system has order n: the entry for n, else
+// the one with the nearest order, else automatic.
+Policy policy_for(const stc_ctx* h, int n, int refine) {
+    const Policy* best = nullptr;
+    int bd = 0x7fffffff;
+    for (const auto& kv : h->policy) {
+        if (kv.first.second != refine) continue;
+        const int d = std::abs(kv.first.first - n);
+        if (d < bd) {
+            bd = d;
+            best = &kv.second;
+        }
+    }
+    return best ? *best : Policy{};
+}
+
 struct LaunchCfg {
     int refine = 0;
     int maxr = 1;
@@ -246,23 +273,26 @@ int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine
     while (maxr < r) maxr <<= 1;
     cfg->refine = refine;
     cfg->maxr = maxr;
+    const Policy pol = policy_for(h, maxn, refine);
     // the register-resident panel holds 4 (or 2, 1) columns per wavefront
-    const int jb_cap = kWaves * panel_cols_per_wave(ts, maxr);
-    int rc = plan_systems(sys, nsys, ts, adj, jb_cap, maxr, &cfg->lds);
+    int jb_cap = kWaves * panel_cols_per_wave(ts, maxr);
+    if (pol.jb_max > 0) jb_cap = std::max(4, std::min(pol.jb_max, jb_cap));
+    int rc = plan_systems(sys, nsys, ts, adj, jb_cap, maxr, pol.kernel == 1 ? pol.lds_budget : 0, &cfg->lds);
     if (rc) return rc;
-    // systems that need a factor slot go to the wide-panel kernel when it is
-    // built for these rows per lane and every system's panel fits its LDS
+    // automatic: systems that need a factor slot go to the wide-panel kernel
+    // when it is built for these rows per lane and every system's panel fits
+    // its LDS; policy kernel 2 takes it for every launch it fits
     bool blocked = false;
     for (int i = 0; i < nsys; ++i) blocked = blocked || !sys[i].lay_small;
     cfg->wide = 0;
-    const char* ew = getenv("STC_WIDE");
-    if (blocked && (!ew || atoi(ew) != 0) && sweep_kernel_fn(refine, maxr, 1)) {
+    if (((pol.kernel == 0 && blocked) || pol.kernel == 2) && sweep_kernel_fn(refine, maxr, 1)) {
         int wcap = wide_jb_max(ts);
-        if (const char* e = getenv("STC_JB_MAX")) wcap = std::max(kSB, std::min(atoi(e), wcap));
-        // two workgroups per CU by default: one per CU leaves the pivot and
-        // solve chains of a single workgroup unhidden
-        int budget = kBudgetSmall;
-        if (const char* e = getenv("STC_WIDE_BUDGET")) budget = std::max(16384, std::min(atoi(e), kBudgetLarge));
+        if (pol.jb_max > 0) wcap = std::max(kSB, std::min(pol.jb_max, wcap));
+        // automatic LDS budget (measured on MI355X, n = 160 .. 1024): two
+        // workgroups per CU up to n = 512 for complex64 and n = 256 for
+        // complex128, one per CU (wider panels, half the slots) beyond
+        const int auto_budget = maxn > (ts == 8 ? 512 : 256) ? kBudgetLarge : kBudgetSmall;
+        const int budget = pol.lds_budget > 0 ? pol.lds_budget : auto_budget;
         int lds = 0;
         if (layout_wide(sys, nsys, ts, adj, wcap, maxr, budget, &lds)) {  // writes only when all fit
             cfg->wide = 1;
@@ -276,7 +306,6 @@ int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfun, kThreads, cfg->lds));
     if (per_cu < 1) per_cu = 1;
     long long g = (long long)h->num_cu * per_cu;
-    if (const char* e = getenv("STC_GRID")) g = std::max(1, atoi(e));
     // every blocked system factors in a per-workgroup slot (also for stc_getrf,
     // which copies the slot out in LAPACK layout afterwards)
     size_t slot = 0;
@@ -483,6 +512,30 @@ int stc_destroy(stc_handle h) {
 }
 
 int stc_device_ok(stc_handle h) { return h ? h->ok_arch : 0; }
+
+int stc_set_policy(stc_handle h, int n, int mode, const stc_policy* p) {
+    if (!h || n < 1 || n > 1024 || check_mode(mode)) return STC_EINVAL;
+    const auto key = std::make_pair(n, mode == STC_REFINED_FP32 ? 1 : 0);
+    if (!p) {
+        h->policy.erase(key);
+        return STC_OK;
+    }
+    if (p->kernel < 0 || p->kernel > 2 || p->jb_max < 0 ||
+        (p->lds_budget != 0 && (p->lds_budget < 16384 || p->lds_budget > kBudgetLarge)))
+        return STC_EINVAL;
+    Policy q;
+    q.kernel = p->kernel;
+    q.lds_budget = p->lds_budget;
+    q.jb_max = p->jb_max;
+    h->policy[key] = q;
+    return STC_OK;
+}
+
+int stc_clear_policies(stc_handle h) {
+    if (!h) return STC_EINVAL;
+    h->policy.clear();
+    return STC_OK;
+}
 
 int stc_rom_set(stc_handle h, int rom_id, int n, const stc_z64* K, const stc_z64* M,
                 const stc_z64* D, int ld, double eta) {

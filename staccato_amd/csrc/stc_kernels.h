@@ -1087,10 +1087,13 @@ __device__ __forceinline__ void tri_solve_pipe(const TriCtx<T>& t, const cx<T>* 
         const int k0 = b * bs, kb = min(bs, n - k0);
 #pragma unroll
         for (int u = 0; u < B; ++u) {
-            const int k = k0 + min(u, kb - 1);
-            // rows outside the update and entries beyond kb are loaded (clamped) but never used
-            r[u] = TRANS ? ldc<1>(LU + ((size_t)lds1<3>(t.rmap + k) + (size_t)i0c * ld))
-                         : ldc<1>(LU + (pr0 + (size_t)k * ld));
+            // rows outside the update are loaded (clamped) but never used; entries
+            // beyond kb (narrow blocks of the register-panel kernel) are not loaded
+            if (u < kb) {
+                const int k = k0 + u;
+                r[u] = TRANS ? ldc<1>(LU + ((size_t)lds1<3>(t.rmap + k) + (size_t)i0c * ld))
+                             : ldc<1>(LU + (pr0 + (size_t)k * ld));
+            }
         }
     };
     cx<T> dc, dn = czero<T>();

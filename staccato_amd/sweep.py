@@ -108,6 +108,21 @@ class Handle:
     def device_ok(self):
         return bool(self.lib.stc_device_ok(self.h))
 
+    # ---- launch policy ------------------------------------------------------------
+    def set_policy(self, n, mode, kernel=0, lds_budget=0, jb_max=0):
+        """Launch policy for systems of order n (or the nearest order with an
+        entry) in `mode`: kernel 0 automatic, 1 register panel, 2 wide LDS
+        panel; lds_budget bytes of LDS per workgroup (0 automatic); jb_max
+        widest panel (0 no cap).  None for kernel removes the entry."""
+        if kernel is None:
+            _check(self.lib.stc_set_policy(self.h, int(n), _mode(mode), None), "stc_set_policy")
+            return
+        p = _lib.Policy(int(kernel), int(lds_budget), int(jb_max))
+        _check(self.lib.stc_set_policy(self.h, int(n), _mode(mode), ctypes.byref(p)), "stc_set_policy")
+
+    def clear_policies(self):
+        _check(self.lib.stc_clear_policies(self.h), "stc_clear_policies")
+
     # ---- operators --------------------------------------------------------------
     def rom_set(self, rom_id, K, M, D=None, eta=0.0):
         K = _fortran_c128(K)

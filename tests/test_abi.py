@@ -47,10 +47,26 @@ def test_strerror_and_invalid_arguments():
     assert lib.stc_device_ok(None) == 0
     assert lib.stc_rom_set(None, 0, 4, None, None, None, 4, 0.0) == _lib.STC_EINVAL
     assert lib.stc_plan_destroy(None) == _lib.STC_EINVAL
+    pol = _lib.Policy(2, 80 * 1024, 0)
+    assert lib.stc_set_policy(None, 256, 1, ctypes.byref(pol)) == _lib.STC_EINVAL
+    assert lib.stc_clear_policies(None) == _lib.STC_EINVAL
+    assert lib.stc_sweep_dev_noalloc(None, 0, 1, None, 1, None, 1, None, 1, None, None, 0, 0.0, 0,
+                                     None) == _lib.STC_EINVAL
     assert lib.stc_gesv_batched(None, 4, 1, None, 4, 16, 1, None, 4, 4, None, 4, 4, None, None,
                                 0, 0.0, 0, None) == _lib.STC_EINVAL
     assert lib.stc_gesv_batched_dev(None, 4, 1, None, 16, 1, None, 4, 4, None, 4, None, None,
                                     0, 0.0, 0, None) == _lib.STC_EINVAL
+
+
+def test_autotune_cache_round_trip(tmp_path):
+    from staccato_amd import autotune
+    path = str(tmp_path / "t.json")
+    assert autotune.load_cache(path) == {}
+    c = {autotune._key(256, 4, "refined"): {"name": "x", "kernel": 2, "lds_budget": 81920,
+                                            "jb_max": 0, "pts_per_s": 1.0}}
+    autotune.save_cache(c, path)
+    assert autotune.load_cache(path) == c
+    assert [k for k, *_ in autotune.CANDIDATES][0] == "register-panel"
 
 
 def test_null_stream_is_refused():
