@@ -32,12 +32,20 @@ TOL_REFINED, TOL_PLAIN = 1e-10, 1e-12
 PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of the sweep kernel on this workload: rocprofv3 --pmc passes
-# (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean over the launches) scaled by what
-# the same counters read for a 256 MiB device-to-device copy (FETCH_SIZE reads half
-# the bytes copied: x2.0; WRITE_SIZE reads them exactly: x1.0).  The summary and the
+# HBM bytes per launch of the sweep kernel on this workload, per launch policy
+# (the autotune candidate the run uses): rocprofv3 --pmc passes (FETCH_SIZE, then
+# WRITE_SIZE; KiB per dispatch, mean over the launches) scaled by what the same
+# counters read for a 256 MiB device-to-device copy (FETCH_SIZE reads half the
+# bytes copied: x2.0; WRITE_SIZE reads them exactly: x1.0).  The summary and the
 # commands are in profiles/sweep_kernel_n256_pmc.txt; re-measure when the kernel changes.
-TRAFFIC_KIB = {"refined": (9.884e6, 14.23e6), "plain": (18.90e6, 34.96e6)}
+TRAFFIC_KIB = {
+    ("refined", "wide-2perCU"): (6.891e6, 8.048e6),
+    ("refined", "wide-1perCU-jb32"): (6.933e6, 8.045e6),
+    ("refined", "register-panel"): (11.50e6, 14.23e6),
+    ("plain", "wide-2perCU"): (14.61e6, 26.95e6),
+    ("plain", "wide-1perCU"): (7.270e6, 14.86e6),
+    ("plain", "register-panel"): (19.67e6, 34.96e6),
+}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
@@ -199,7 +207,7 @@ def pts_total(cases):
     return sum(c.freq.size for c in cases)
 
 
-def other_workload(name, steps, warmup):
+def other_workload(name, steps, warmup, policy=""):
     """Throughput of the other configurations (reported beside the headline,
     never as its value): the uneven 64-ROM batch (hipGraph plan), the n=128
     adjoint and the n=1024 dense sweep, all refined FP32 unless stated."""
@@ -289,12 +297,19 @@ def other_workload(name, steps, warmup):
         stream = torch.cuda.Stream(device=dev)
         from staccato_amd import autotune
         out["autotune"] = {}
+        fixed = dict(kv.split("=", 1) for kv in policy.split(",") if "=" in kv)
+        cands = {q[0]: q for q in autotune.CANDIDATES}
         for mode in ("refined", "plain"):
             tol = TOL_REFINED if mode == "refined" else TOL_PLAIN
-            best, cand = autotune.tune(h, 0, c.freq, c.F, mode, tol, sample=c.freq.size, reps=1,
-                                       cache=False)
-            out["autotune"][mode] = {"chosen": best["name"] if best else "automatic",
-                                     "candidates_pts_per_s": {q["name"]: round(q["pts_per_s"]) for q in cand}}
+            if mode in fixed:
+                _, kernel, budget, jb = cands[fixed[mode]]
+                h.set_policy(1024, mode, kernel, budget, jb)
+                out["autotune"][mode] = {"chosen": fixed[mode], "given": True}
+            else:
+                best, cand = autotune.tune(h, 0, c.freq, c.F, mode, tol, sample=c.freq.size, reps=1,
+                                           cache=False)
+                out["autotune"][mode] = {"chosen": best["name"] if best else "automatic",
+                                         "candidates_pts_per_s": {q["name"]: round(q["pts_per_s"]) for q in cand}}
             run = lambda: h.sweep_dev(0, fr, F, X, info, be if mode == "refined" else None,
                                       mode=mode, tol=TOL_REFINED, max_iter=10,
                                       stream=stream.cuda_stream)
@@ -311,6 +326,35 @@ def other_workload(name, steps, warmup):
                          "mfma_peak_frac": tfl / peak,
                          "max_backward_error": float(be.max()),
                          "points_failed": int((info != 0).sum()),
+                         "max_rel_error_vs_closed_form": float(fe.max())}
+    elif name == "small60":
+        # the reference's CPU-runnable case: one n=60 ROM, 1000 frequencies, a unit
+        # force at DoF 0; the whole matrix is factored in LDS (no factor slot)
+        c = synth.single_n60()
+        rom = c.rom
+        h.rom_set(0, rom.K, rom.M, rom.D, rom.eta)
+        fr = torch.tensor(c.freq, dtype=torch.float64, device=dev)
+        F = torch.tensor(np.ascontiguousarray(c.F.T), dtype=torch.complex128, device=dev)
+        nf, nr = c.freq.size, c.F.shape[1]
+        X = torch.empty((nf, nr, 60), dtype=torch.complex128, device=dev)
+        info = torch.empty(nf, dtype=torch.int32, device=dev)
+        be = torch.empty((nf, nr), dtype=torch.float64, device=dev)
+        stream = torch.cuda.Stream(device=dev)
+        for mode in ("plain", "refined"):
+            tol = TOL_REFINED if mode == "refined" else TOL_PLAIN
+            run = lambda: h.sweep_dev(0, fr, F, X, info, be, mode=mode, tol=tol, max_iter=10,
+                                      stream=stream.cuda_stream)
+            wall, ev = time_gpu(run, steps, warmup, stream)
+            torch.cuda.synchronize()
+            Xh = X.cpu().numpy().transpose(0, 2, 1)
+            Xc = synth.closed_form(rom, c.freq, c.F)
+            fe = np.abs(Xh - Xc).max(axis=(1, 2)) / np.abs(Xc).max(axis=(1, 2))
+            # algorithmic HBM bytes per sweep: operators once, F in, X out, info and berr
+            byts = 3 * 60 * 60 * 8 + nr * 60 * 16 + nf * (nr * 60 * 16 + 4 + nr * 8)
+            out[mode] = {"pts_per_s": nf * steps / wall, "kernel_ms": 1e3 * ev / steps,
+                         "algorithmic_hbm_gbs": byts * steps / ev / 1e9,
+                         "tflops": nf * work_per_point(60, nr) * steps / ev / 1e12,
+                         "max_backward_error": float(be.max()), "points_failed": int((info != 0).sum()),
                          "max_rel_error_vs_closed_form": float(fe.max())}
     elif name == "ijcsa":
         # interface-Jacobian coupling: a chain of three n=256 substructures
@@ -387,8 +431,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="both", choices=["both", "refined", "plain"])
     ap.add_argument("--workload", default="uniform",
-                    choices=["uniform", "uneven", "adjoint", "dense1024", "ijcsa"],
+                    choices=["uniform", "uneven", "adjoint", "dense1024", "small60", "ijcsa"],
                     help="uniform = the headline n=256 sweep; the others print their own line")
+    ap.add_argument("--policy", default="", metavar="MODE=NAME[,MODE=NAME]",
+                    help="use these autotune candidates instead of tuning (e.g. for a profile "
+                         "of exactly the kernels a tuned run uses)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (a kernel of known "
                          "traffic, to scale rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -400,7 +447,7 @@ def main():
         sys.exit(2)
 
     if args.workload != "uniform":
-        other_workload(args.workload, args.steps, args.warmup)
+        other_workload(args.workload, args.steps, args.warmup, args.policy)
         return
 
     import torch
@@ -449,12 +496,20 @@ def main():
     # launch policy per mode picked by measurement (staccato_amd.autotune): every
     # candidate timed on the full sweep, the fastest whose points all meet tol
     tuned = {}
+    fixed = dict(kv.split("=", 1) for kv in args.policy.split(",") if "=" in kv)
+    cands = {c[0]: c for c in autotune.CANDIDATES}
     for mode, tol in (("refined", TOL_REFINED), ("plain", TOL_PLAIN)):
-        if args.mode in ("both", mode):
-            best, cand = autotune.tune(h, 0, case.freq, case.F, mode, tol, sample=NFREQ, reps=2,
-                                       cache=False)
-            tuned[mode] = {"chosen": best["name"] if best else "automatic",
-                           "candidates_pts_per_s": {c["name"]: round(c["pts_per_s"]) for c in cand}}
+        if args.mode not in ("both", mode):
+            continue
+        if mode in fixed:
+            _, kernel, budget, jb = cands[fixed[mode]]
+            h.set_policy(N, mode, kernel, budget, jb)
+            tuned[mode] = {"chosen": fixed[mode], "given": True}
+            continue
+        best, cand = autotune.tune(h, 0, case.freq, case.F, mode, tol, sample=NFREQ, reps=2,
+                                   cache=False)
+        tuned[mode] = {"chosen": best["name"] if best else "automatic",
+                       "candidates_pts_per_s": {c["name"]: round(c["pts_per_s"]) for c in cand}}
 
     res = {}
     for mode, tol in (("refined", TOL_REFINED), ("plain", TOL_PLAIN)):
@@ -472,8 +527,12 @@ def main():
     r = res[main_mode]
     peak = PEAK_FP32_TFLOPS if main_mode == "refined" else PEAK_FP64_TFLOPS
     achieved = W * NFREQ / (r["kernel_ms"] * 1e-3) / 1e12
-    fk, wk = TRAFFIC_KIB[main_mode]
-    traffic = (fk * FETCH_SCALE + wk * WRITE_SCALE) * 1024.0
+    chosen = tuned.get(main_mode, {}).get("chosen", "")
+    if (main_mode, chosen) in TRAFFIC_KIB:
+        fk, wk = TRAFFIC_KIB[(main_mode, chosen)]
+        traffic = (fk * FETCH_SCALE + wk * WRITE_SCALE) * 1024.0
+    else:
+        traffic = None
     out = {
         "metric": "freq points solved/s at n=256 vs plain FP64 GPU LU (rel. residual <=1e-10)",
         "value": r["pts_per_s"],
@@ -493,7 +552,8 @@ def main():
                    "tol": TOL_REFINED if main_mode == "refined" else TOL_PLAIN},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
-                     "traffic_unit": "bytes per launch (HBM, rocprof FETCH_SIZE/WRITE_SIZE)",
+                     "traffic_unit": "bytes per launch (HBM, rocprof FETCH_SIZE/WRITE_SIZE) of "
+                                     "the launch policy the run uses",
                      "work_per_point_flop": W, "points_per_launch": NFREQ},
         "results": res,
         "autotune": tuned,

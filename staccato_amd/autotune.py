@@ -69,12 +69,22 @@ def apply_cached(handle, n, nrhs, mode, cache=None):
     return True
 
 
+# The automatic policy of the library for an order n and mode (kept unless a
+# candidate is faster by more than `margin`: near-ties are noise).
+def automatic_name(n, mode):
+    complex64 = mode in ("refined", "fp32", 1)
+    if n <= 128:
+        return "register-panel"
+    return "wide-1perCU" if n > (512 if complex64 else 256) else "wide-2perCU"
+
+
 def tune(handle, rom_id, freq_hz, F, mode, tol, sample=1024, reps=3, candidates=None,
-         cache=None, cache_path=None, verbose=False):
+         cache=None, cache_path=None, verbose=False, margin=0.02):
     """Time each candidate policy on `sample` of the frequencies (evenly
     spaced) with load F (n x nrhs), install the fastest that meets tol at every
-    point, record it (cache_path, or the default cache; cache=False: do not
-    record) and return (best_entry, all_results)."""
+    point (the automatic policy unless another is faster by more than margin),
+    record it (cache_path, or the default cache; cache=False: do not record)
+    and return (best_entry, all_results)."""
     import torch
     n = handle.n[rom_id]
     F = np.asarray(F, dtype=np.complex128)
@@ -118,6 +128,9 @@ def tune(handle, rom_id, freq_hz, F, mode, tol, sample=1024, reps=3, candidates=
         handle.set_policy(n, mode, None)
         return None, results
     best = max(good, key=lambda r: r["pts_per_s"])
+    auto = [r for r in good if r["name"] == automatic_name(n, mode)]
+    if auto and best["pts_per_s"] <= auto[0]["pts_per_s"] * (1.0 + margin):
+        best = auto[0]
     handle.set_policy(n, mode, best["kernel"], best["lds_budget"], best["jb_max"])
     if cache is not False:
         c = load_cache(cache_path) if cache is None else cache

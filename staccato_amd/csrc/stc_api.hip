@@ -280,12 +280,14 @@ int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine
     int rc = plan_systems(sys, nsys, ts, adj, jb_cap, maxr, pol.kernel == 1 ? pol.lds_budget : 0, &cfg->lds);
     if (rc) return rc;
     // automatic: systems that need a factor slot go to the wide-panel kernel
-    // when it is built for these rows per lane and every system's panel fits
-    // its LDS; policy kernel 2 takes it for every launch it fits
+    // when the largest order exceeds 128 (below, the register panel at three
+    // workgroups per CU measured faster), the kernel is built for these rows
+    // per lane and every system's panel fits its LDS; policy kernel 2 takes it
+    // for every launch it fits
     bool blocked = false;
     for (int i = 0; i < nsys; ++i) blocked = blocked || !sys[i].lay_small;
     cfg->wide = 0;
-    if (((pol.kernel == 0 && blocked) || pol.kernel == 2) && sweep_kernel_fn(refine, maxr, 1)) {
+    if (((pol.kernel == 0 && blocked && maxn > 128) || pol.kernel == 2) && sweep_kernel_fn(refine, maxr, 1)) {
         int wcap = wide_jb_max(ts);
         if (pol.jb_max > 0) wcap = std::max(kSB, std::min(pol.jb_max, wcap));
         // automatic LDS budget (measured on MI355X, n = 160 .. 1024): two

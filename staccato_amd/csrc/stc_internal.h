@@ -184,14 +184,15 @@ constexpr int kSB = 16;  // sub-panel width and inverted diagonal block size
 // 64 columns for complex64, 32 for complex128.
 __host__ __device__ constexpr int wide_jb_max(int elem_bytes) { return elem_bytes == 8 ? 64 : 32; }
 
-// LDS of the wide path beyond lds_fixed_bytes: the n x jb panel, its inverted
+// LDS of the wide path beyond lds_fixed_bytes: the n x jb panel (leading
+// dimension rounded up to odd, so its columns start in different banks), its inverted
 // 16 x 16 diagonal blocks, the sub-panel's two alternating pivot-candidate
 // buffers (per wave one row of 16 and its 16-byte key) and the moved-row
 // list (2 x 16).  maxr is not needed (kept for the callers' signature).
 __host__ __device__ inline long long wide_cand_bytes(int ts) { return 2LL * kWaves * kSB * ts + 2LL * kWaves * 16; }
 __host__ __device__ inline long long wide_lds_bytes(int n, int jb, int ts, int maxr) {
     (void)maxr;
-    return align16((long long)n * jb * ts) + align16((long long)jb * kSB * ts) + align16(wide_cand_bytes(ts)) +
+    return align16((long long)(n | 1) * jb * ts) + align16((long long)jb * kSB * ts) + align16(wide_cand_bytes(ts)) +
            align16(2LL * kSB * 8);
 }
 

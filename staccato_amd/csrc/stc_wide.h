@@ -119,12 +119,12 @@ __device__ __forceinline__ void stc1u(unsigned* p, unsigned v) { *(__attribute__
 // info = first zero column.  ipiv[j0 + c0 + jj] = 0-based pivot row;
 // sc[1] = first zero pivot (1-based).
 template <typename T, int MAXR>
-__device__ __forceinline__ void subpanel_factor(cx<T>* __restrict__ P, int m, int jbl, int c0, int sbw, int j0,
+__device__ __forceinline__ void subpanel_factor(cx<T>* __restrict__ P, int m, int ldp, int jbl, int c0, int sbw, int j0,
                                                 int* ipiv, int* rmap, int* sc, cx<T>* rowbuf, int2* mvl) {
     constexpr int RM = MAXR >= 4 ? MAXR / 4 : 1;  // rows per thread
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int mq = m - c0;
-    cx<T>* Q = P + c0 + (size_t)c0 * m;  // sub-panel origin, ld m
+    cx<T>* Q = P + c0 + (size_t)c0 * ldp;  // sub-panel origin, ld ldp
     // candidate keys after the row buffers: 2 buffers x 4 waves x {key lo, key hi, lo, -}
     unsigned* cand = reinterpret_cast<unsigned*>(rowbuf + 2 * kWaves * kSB);
     cx<T> x[kSB][RM];
@@ -135,7 +135,7 @@ __device__ __forceinline__ void subpanel_factor(cx<T>* __restrict__ P, int m, in
         lpos[t] = r < mq ? r : 0x7fffffff;
 #pragma unroll
         for (int c = 0; c < kSB; ++c)
-            x[c][t] = ldz<3>(Q + (min(r, mq - 1) + (size_t)min(c, sbw - 1) * m), c < sbw && r < mq);
+            x[c][t] = ldz<3>(Q + (min(r, mq - 1) + (size_t)min(c, sbw - 1) * ldp), c < sbw && r < mq);
     }
     if (tid == 0) stc1i(sc + 12, 0);
     for (int jj = 0; jj < sbw; ++jj) {
@@ -243,7 +243,7 @@ __device__ __forceinline__ void subpanel_factor(cx<T>* __restrict__ P, int m, in
         if (r < mq) {
 #pragma unroll
             for (int c = 0; c < kSB; ++c)
-                if (c < sbw) stc<3>(Q + (lpos[t] + (size_t)c * m), x[c][t]);
+                if (c < sbw) stc<3>(Q + (lpos[t] + (size_t)c * ldp), x[c][t]);
             if (lpos[t] != r) {
                 const int idx = atomicAdd(sc + 12, 1);
                 mvl[idx] = make_int2(c0 + r, c0 + lpos[t]);
@@ -270,7 +270,7 @@ __device__ __forceinline__ void subpanel_factor(cx<T>* __restrict__ P, int m, in
                 ti[u] = rmap[j0 + mvp.x];
             } else {
                 const int c = k < c0 ? k : k + sbw;
-                tv[u] = ldc<3>(P + (mvp.x + (size_t)c * m));
+                tv[u] = ldc<3>(P + (mvp.x + (size_t)c * ldp));
             }
         }
     }
@@ -285,7 +285,7 @@ __device__ __forceinline__ void subpanel_factor(cx<T>* __restrict__ P, int m, in
                 rmap[j0 + mvp.y] = ti[u];
             } else {
                 const int c = k < c0 ? k : k + sbw;
-                stc<3>(P + (mvp.y + (size_t)c * m), tv[u]);
+                stc<3>(P + (mvp.y + (size_t)c * ldp), tv[u]);
             }
         }
     }
@@ -325,7 +325,8 @@ __device__ __forceinline__ void invert_unit_lower16(const cx<T>* Q, int ldq, int
 // Rows [c0, c0+16): U = Linv A (the block row of U); rows [c0+16, m):
 // A -= L U.  Columns [c0 + 16, jbl) in chunks of 16.
 template <typename T>
-__device__ __forceinline__ void inpanel_update(cx<T>* __restrict__ P, int m, int jbl, int c0, const cx<T>* Linv) {
+__device__ __forceinline__ void inpanel_update(cx<T>* __restrict__ P, int m, int ldp, int jbl, int c0,
+                                               const cx<T>* Linv) {
     using M = Mfma<T>;
     using acc_t = typename M::acc;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -342,13 +343,13 @@ __device__ __forceinline__ void inpanel_update(cx<T>* __restrict__ P, int m, int
         for (int s = 0; s < 4; ++s) {
             const int k = M::row(lg, s);
             const cx<T> a = ldc<3>(Linv + (li + k * kSB));
-            const cx<T> b = ldz<3>(P + ((c0 + k) + (size_t)(cb + min(li, cw - 1)) * m), li < cw);
+            const cx<T> b = ldz<3>(P + ((c0 + k) + (size_t)(cb + min(li, cw - 1)) * ldp), li < cw);
             cmfma<T>(ure, uim, a, b.re, b.im);
         }
         if (li < cw)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                stc<3>(P + ((c0 + M::row(lg, r)) + (size_t)(cb + li) * m), cmk<T>(ure[r], uim[r]));
+                stc<3>(P + ((c0 + M::row(lg, r)) + (size_t)(cb + li) * ldp), cmk<T>(ure[r], uim[r]));
     }
     __syncthreads();
     // rows below: (row tile, chunk) units round-robin over the waves
@@ -361,7 +362,7 @@ __device__ __forceinline__ void inpanel_update(cx<T>* __restrict__ P, int m, int
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = r0 + M::row(lg, r);
-            const cx<T> v = ldz<3>(P + (min(row, m - 1) + (size_t)(cb + min(li, cw - 1)) * m), row < m && li < cw);
+            const cx<T> v = ldz<3>(P + (min(row, m - 1) + (size_t)(cb + min(li, cw - 1)) * ldp), row < m && li < cw);
             dre[r] = v.re;
             dim[r] = v.im;
         }
@@ -369,16 +370,16 @@ __device__ __forceinline__ void inpanel_update(cx<T>* __restrict__ P, int m, int
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int k = M::row(lg, s);
-            cx<T> a = ldc<3>(P + (arow + (size_t)(c0 + k) * m));
+            cx<T> a = ldc<3>(P + (arow + (size_t)(c0 + k) * ldp));
             a.re = -a.re;
             a.im = -a.im;
-            const cx<T> b = ldz<3>(P + ((c0 + k) + (size_t)(cb + min(li, cw - 1)) * m), li < cw);
+            const cx<T> b = ldz<3>(P + ((c0 + k) + (size_t)(cb + min(li, cw - 1)) * ldp), li < cw);
             cmfma<T>(dre, dim, a, b.re, b.im);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = r0 + M::row(lg, r);
-            if (row < m && li < cw) stc<3>(P + (row + (size_t)(cb + li) * m), cmk<T>(dre[r], dim[r]));
+            if (row < m && li < cw) stc<3>(P + (row + (size_t)(cb + li) * ldp), cmk<T>(dre[r], dim[r]));
         }
     }
     __syncthreads();
@@ -386,7 +387,7 @@ __device__ __forceinline__ void inpanel_update(cx<T>* __restrict__ P, int m, int
 
 // ---- trailing update after a wide panel ------------------------------------------------------
 // Columns [j0 + jbl, ncol) (the matrix, then right-hand sides), chunks of 16
-// columns dealt to the waves.  P: the factored panel (LDS, ld ldp = n - j0,
+// columns dealt to the waves.  P: the factored panel (LDS, ld ldp >= n - j0,
 // logical order); Linv: its inverted 16 x 16 diagonal blocks (ld 16).
 // ASM: first panel, whose trailing entries are assembled from the operators
 // (physical row r, column c of A(w) is element r + c n) instead of loaded.
@@ -547,7 +548,7 @@ __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w
     const int jb = lay.jb;
     char* base = L.main;
     cx<T>* P = reinterpret_cast<cx<T>*>(base);
-    base += align16((long long)n * jb * sizeof(cx<T>));
+    base += align16((long long)(n | 1) * jb * sizeof(cx<T>));
     cx<T>* Linv = reinterpret_cast<cx<T>*>(base);
     base += align16((long long)jb * kSB * sizeof(cx<T>));
     cx<T>* cand = reinterpret_cast<cx<T>*>(base);  // sub-panel pivot candidates
@@ -569,6 +570,9 @@ __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w
 
     for (int j0 = 0; j0 < n; j0 += jb) {
         const int jbl = min(jb, n - j0), m = n - j0;
+        // complex64: odd leading dimension, so the columns an MFMA operand read
+        // spans start in different LDS banks (complex128 elements spread already)
+        const int ldp = sizeof(T) == 4 ? (m | 1) : m;
         // ---- panel into LDS (logical rows), assembled from the operators at j0 = 0;
         // every thread owns rows and keeps kPB columns of loads in flight
         for (int r = tid; r < m; r += kThreads) {
@@ -583,30 +587,30 @@ __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w
                 }
 #pragma unroll
                 for (int u = 0; u < kPB; ++u)
-                    if (cb + u < jbl) stc<3>(P + (r + (size_t)(cb + u) * m), v[u]);
+                    if (cb + u < jbl) stc<3>(P + (r + (size_t)(cb + u) * ldp), v[u]);
             }
             // columns from jbl to the next multiple of 16 read as zero by the
             // trailing update's last k-block
-            for (int c = jbl; c < ((jbl + kSB - 1) / kSB) * kSB; ++c) stc<3>(P + (r + (size_t)c * m), czero<T>());
+            for (int c = jbl; c < ((jbl + kSB - 1) / kSB) * kSB; ++c) stc<3>(P + (r + (size_t)c * ldp), czero<T>());
         }
         __syncthreads();
         // ---- sub-panels
         for (int c0 = 0; c0 < jbl; c0 += kSB) {
             const int sbw = min(kSB, jbl - c0);
-            subpanel_factor<T, MAXR>(P, m, jbl, c0, sbw, j0, L.ipiv, L.rmap, L.sc, cand, mvl);
-            invert_unit_lower16<T>(P + c0 + (size_t)c0 * m, m, sbw, Linv + (size_t)(c0 / kSB) * kSB * kSB,
+            subpanel_factor<T, MAXR>(P, m, ldp, jbl, c0, sbw, j0, L.ipiv, L.rmap, L.sc, cand, mvl);
+            invert_unit_lower16<T>(P + c0 + (size_t)c0 * ldp, ldp, sbw, Linv + (size_t)(c0 / kSB) * kSB * kSB,
                                    Lg + (size_t)((j0 + c0) / kSB) * kSB * kSB);
             __syncthreads();
-            if (c0 + kSB < jbl) inpanel_update<T>(P, m, jbl, c0, Linv + (size_t)(c0 / kSB) * kSB * kSB);
+            if (c0 + kSB < jbl) inpanel_update<T>(P, m, ldp, jbl, c0, Linv + (size_t)(c0 / kSB) * kSB * kSB);
         }
         // ---- the factored panel to the slot (rows at their physical positions)
         for (int r = tid; r < m; r += kThreads) {
             const int pr = lds1<3>(L.rmap + j0 + r);
-            for (int c = 0; c < jbl; ++c) stc<1>(S + (pr + (size_t)(j0 + c) * n), ldc<3>(P + (r + (size_t)c * m)));
+            for (int c = 0; c < jbl; ++c) stc<1>(S + (pr + (size_t)(j0 + c) * n), ldc<3>(P + (r + (size_t)c * ldp)));
         }
         // ---- trailing columns
-        if (j0 == 0) wide_trailing<T, true>(s, w, w2, S, n, L.rmap, j0, jbl, n, n + nr, P, m, Linv);
-        else wide_trailing<T, false>(s, w, w2, S, n, L.rmap, j0, jbl, n, n + nr, P, m, Linv);
+        if (j0 == 0) wide_trailing<T, true>(s, w, w2, S, n, L.rmap, j0, jbl, n, n + nr, P, ldp, Linv);
+        else wide_trailing<T, false>(s, w, w2, S, n, L.rmap, j0, jbl, n, n + nr, P, ldp, Linv);
         __syncthreads();
     }
     if (sizeof(T) == 4) {
