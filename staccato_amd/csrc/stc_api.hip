@@ -105,7 +105,7 @@ struct stc_ctx {
     int live_plans = 0;
     std::map<std::pair<int, int>, Policy> policy;  // (order n, refine) -> policy
     // staging for host-memory calls
-    DevBuf freq, F, X, info, berr, iters, fac, ipiv, sys, items, adj, c, phi, kd, A;
+    DevBuf freq, F, X, info, berr, iters, fac, ipiv, sys, items, adj, c, phi, kd, A, load;
 };
 
 struct stc_plan_s {
@@ -418,23 +418,26 @@ int upload_block(DevBuf& b, const stc_z64* src, int n, int ncol, int ld, hipStre
     return STC_OK;
 }
 
-// Y(o, j) = sum_i C(o, i) X(i, j) for every frequency block (batched projection).
-__global__ void project_kernel(const double2* C, int nout, int n, const double2* X, int ncol,
-                               double2* Y) {
-    const int f = blockIdx.x;
-    const double2* Xf = X + (size_t)f * n * ncol;
-    double2* Yf = Y + (size_t)f * nout * ncol;
-    for (int e = threadIdx.x; e < nout * ncol; e += blockDim.x) {
-        const int o = e % nout, j = e / nout;
-        double re = 0.0, im = 0.0;
-        for (int i = 0; i < n; ++i) {
-            const double2 c = C[o + (size_t)i * nout];
-            const double2 x = Xf[i + (size_t)j * n];
-            re += c.x * x.x - c.y * x.y;
-            im += c.x * x.y + c.y * x.x;
-        }
-        Yf[e] = make_double2(re, im);
+// Right-hand sides of the back-transformation, formed on the device:
+// F(:, j) = B L(:, j) for j < nrhs (B n x nin, ld n; L nin x nrhs, ld nin) and,
+// when withB, F(:, nrhs + k) = B(:, k) (the columns of the transfer function).
+__global__ void load_rhs_kernel(const double2* B, int n, int nin, const double2* L, int nrhs,
+                                int withB, double2* F) {
+    const long long ncol = nrhs + (withB ? nin : 0);
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * ncol) return;
+    const int i = (int)(e % n), j = (int)(e / n);
+    if (j >= nrhs) {
+        F[e] = B[i + (size_t)(j - nrhs) * n];
+        return;
     }
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < nin; ++k) {
+        const double2 b = B[i + (size_t)k * n], l = L[k + (size_t)j * nin];
+        re = fma(b.x, l.x, fma(-b.y, l.y, re));
+        im = fma(b.x, l.y, fma(b.y, l.x, im));
+    }
+    F[e] = make_double2(re, im);
 }
 
 }  // namespace
@@ -505,7 +508,7 @@ int stc_destroy(stc_handle h) {
         }
         DevBuf* bufs[] = {&h->scratch, &h->counter, &h->freq, &h->F, &h->X, &h->info, &h->berr,
                           &h->iters, &h->fac, &h->ipiv, &h->sys, &h->items, &h->adj, &h->c,
-                          &h->phi, &h->kd, &h->A};
+                          &h->phi, &h->kd, &h->A, &h->load};
         for (DevBuf* b : bufs) b->release();
         (void)hipStreamDestroy(h->stream);
     }
@@ -1132,36 +1135,35 @@ int stc_back_transform(stc_handle h, int rom_id, int nfreq, const double* freq_h
     const int ncol = nrhs + (H ? nin : 0);
     DeviceGuard g(h->device);
     hipStream_t st = h->stream;
-    // right-hand sides [B_R * inputLoad | B_R] (the projection is frequency independent,
-    // hoisted out of the loop of KrylovROMSubstructure.cpp:1423-1424)
-    std::vector<stc_z64> Bh((size_t)n * nin);
-    HIP_TRY(hipMemcpy(Bh.data(), r.B.p, Bh.size() * sizeof(stc_z64), hipMemcpyDeviceToHost));
-    std::vector<stc_z64> rhs((size_t)n * ncol);
-    for (int j = 0; j < nrhs; ++j)
-        for (int i = 0; i < n; ++i) {
-            double re = 0.0, im = 0.0;
-            for (int k = 0; k < nin; ++k) {
-                const stc_z64 b = Bh[(size_t)k * n + i];
-                const stc_z64 l = input_load[(size_t)j * nin + k];
-                re += b.re * l.re - b.im * l.im;
-                im += b.re * l.im + b.im * l.re;
-            }
-            rhs[(size_t)j * n + i] = {re, im};
-        }
-    if (H) memcpy(&rhs[(size_t)nrhs * n], Bh.data(), Bh.size() * sizeof(stc_z64));
     int rc;
+    // right-hand sides [B_R * inputLoad | B_R] on the device (the projection is
+    // frequency independent, hoisted out of the loop of
+    // KrylovROMSubstructure.cpp:1423-1424)
     if ((rc = upload(h->freq, freq_hz, sizeof(double) * nfreq, st))) return rc;
-    if ((rc = upload(h->F, rhs.data(), rhs.size() * sizeof(stc_z64), st))) return rc;
-    if ((rc = h->X.ensure(sizeof(stc_z64) * (size_t)nfreq * n * ncol))) return rc;
+    if ((rc = upload(h->load, input_load, sizeof(stc_z64) * (size_t)nin * nrhs, st))) return rc;
+    if ((rc = h->F.ensure(sizeof(stc_z64) * (size_t)n * ncol))) return rc;
     if ((rc = h->info.ensure(sizeof(int) * nfreq))) return rc;
     if ((rc = h->berr.ensure(sizeof(stc_z64) * (size_t)nfreq * nout * ncol))) return rc;
+    {
+        const long long cnt = (long long)n * ncol;
+        hipLaunchKernelGGL(load_rhs_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st,
+                           r.B.as<double2>(), n, nin, h->load.as<double2>(), nrhs, H ? 1 : 0,
+                           h->F.as<double2>());
+        HIP_TRY(hipGetLastError());
+    }
+    // y = C_R X and H_R = C_R A^{-1} B_R (:1438-1448), formed by the sweep kernel
+    // from each point's solution; X itself is never written to memory
+    double2* Yd = h->berr.as<double2>();
     KParams kp = base_params(mode, tol, max_iter);
     std::vector<SysDesc> sys(1, base_desc(r));
     SysDesc& s = sys[0];
     s.nrhs = ncol; s.nfreq = nfreq; s.ldf = n; s.ldx = n;
     s.freq = h->freq.as<double>();
     s.F = h->F.as<double2>();
-    s.X = h->X.as<double2>();
+    s.X = nullptr;
+    s.C = r.C.as<double2>();
+    s.Y = Yd;
+    s.nout = nout;
     s.info = h->info.as<int>();
     LaunchCfg cfg;
     if ((rc = configure(h, sys, nfreq, mode == STC_REFINED_FP32, 0, &cfg))) return rc;
@@ -1172,12 +1174,6 @@ int stc_back_transform(stc_handle h, int rom_id, int nfreq, const double* freq_h
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, st));
     if ((rc = enqueue(h, cfg, kp, st))) return rc;
-    // outputs y = C_R X and H_R = C_R A^{-1} B_R for every frequency (:1438-1448)
-    double2* Yd = h->berr.as<double2>();
-    if ((size_t)nfreq * nout * ncol * sizeof(double2) > h->berr.bytes) return STC_EINVAL;
-    hipLaunchKernelGGL(project_kernel, dim3(nfreq), dim3(256), 0, st, r.C.as<double2>(), nout, n,
-                       h->X.as<double2>(), ncol, Yd);
-    HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipStreamSynchronize(st));
     float ms = 0.f;

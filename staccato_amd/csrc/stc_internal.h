@@ -54,6 +54,12 @@ struct SysDesc {
     const double* Phi;   // n x p, ld = n
     const double* kdiag; // p
     double* adj_out;     // nfreq blocks of (p + 2): J_w, dJ/deta_w, dJ/ds_k,w
+    // fused output projection: C nout x n (ld nout), Y nfreq blocks of
+    // nout x nrhs (ld nout); X may then be null (not written)
+    const double2* C;
+    double2* Y;
+    int nout;
+    int pad_;
     long long item_base; // first point index of this system in the queue
     // explicit matrices: > 0 makes K, M and D advance by op_stride doubles and F by
     // f_stride complex entries per frequency index (freq == nullptr: w = 0)

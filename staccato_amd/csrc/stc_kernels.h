@@ -1537,6 +1537,63 @@ __device__ __forceinline__ void adjoint_point(const SysDesc& s, int fidx, double
 
 namespace stc {
 
+// ---- fused output projection -------------------------------------------------------------
+// Y(:, g0 + g) = C x_g for the gc solved columns x_g = xs + g n (FP64) of one
+// point: nout x gc outputs, each a length-n complex dot product with the
+// nout x n matrix C (ld nout).  Many outputs: one per thread, the reads of C
+// coalesced over o.  Few outputs (nout << n): one per wavefront, its lanes
+// split the sum over i and reduce by butterfly.  Only Y leaves the workgroup.
+__device__ __forceinline__ void project_cols(const SysDesc& s, int fidx, const z64* xs, int g0, int gc) {
+    const int n = s.n, nout = s.nout, tid = threadIdx.x, q = nout * gc;
+    double2* Y = s.Y + ((size_t)fidx * s.nrhs + g0) * nout;
+    if (q >= kThreads / 2) {
+        for (int e = tid; e < q; e += kThreads) {
+            const int o = e % nout;
+            const z64* x = xs + (size_t)(e / nout) * n;
+            double re = 0.0, im = 0.0;
+            for (int i = 0; i < n; ++i) {
+                const double2 c = s.C[o + (size_t)i * nout];
+                const z64 v = x[i];
+                re = fma(c.x, v.re, fma(-c.y, v.im, re));
+                im = fma(c.x, v.im, fma(c.y, v.re, im));
+            }
+            Y[e] = make_double2(re, im);
+        }
+    } else {
+        const int lane = tid & 63;
+        for (int e = tid >> 6; e < q; e += kWaves) {
+            const int o = e % nout;
+            const z64* x = xs + (size_t)(e / nout) * n;
+            double re = 0.0, im = 0.0;
+            for (int i = lane; i < n; i += 64) {
+                const double2 c = s.C[o + (size_t)i * nout];
+                const z64 v = x[i];
+                re = fma(c.x, v.re, fma(-c.y, v.im, re));
+                im = fma(c.x, v.im, fma(c.y, v.re, im));
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                re += __shfl_xor(re, off, 64);
+                im += __shfl_xor(im, off, 64);
+            }
+            if (lane == 0) Y[e] = make_double2(re, im);
+        }
+    }
+}
+
+// NaN into X and Y of a point that could not be solved.
+__device__ __forceinline__ void nan_outputs(const SysDesc& s, int fidx) {
+    const int tid = threadIdx.x;
+    const double2 qn = make_double2(__builtin_nan(""), __builtin_nan(""));
+    if (s.X) {
+        double2* X = s.X + (size_t)fidx * s.ldx * s.nrhs;
+        for (int e = tid; e < s.n * s.nrhs; e += kThreads) X[e % s.n + (size_t)(e / s.n) * s.ldx] = qn;
+    }
+    if (s.Y) {
+        double2* Y = s.Y + (size_t)fidx * s.nout * s.nrhs;
+        for (int e = tid; e < s.nout * s.nrhs; e += kThreads) Y[e] = qn;
+    }
+}
+
 // ---- one point ---------------------------------------------------------------------------
 // WIDE: blocked systems use the wide-panel factorisation (factor_wide, LDS
 // panel of lay.jb columns, 16 x 16 inverted diagonal blocks); otherwise the
@@ -1574,12 +1631,7 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
         if (kp.scratch == nullptr || kp.slot_elems < slot_elems_for(n, WIDE ? kSB : lay.jb) || bad_jb) {
             // no factor slot, or a panel wider than the register panel (the host
             // plans neither): report, write NaN, touch nothing else
-            double2* Xn = s.X ? s.X + (size_t)fidx * s.ldx * nrhs : nullptr;
-            if (Xn)
-                for (int e = tid; e < n * nrhs; e += kThreads) {
-                    const int r = e % n, g = e / n;
-                    Xn[r + (size_t)g * s.ldx] = make_double2(__builtin_nan(""), __builtin_nan(""));
-                }
+            nan_outputs(s, fidx);
             if (tid == 0) s.info[fidx] = -2;
             __syncthreads();
             return;
@@ -1610,14 +1662,9 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
         return;
     }
 
-    double2* X = s.X + (size_t)fidx * s.ldx * nrhs;
+    double2* X = s.X ? s.X + (size_t)fidx * s.ldx * nrhs : nullptr;
     if (info != 0) {
-        const double qnan = __builtin_nan("");
-        if (X)
-            for (int e = tid; e < n * nrhs; e += kThreads) {
-                const int r = e % n, g = e / n;
-                X[r + (size_t)g * s.ldx] = make_double2(qnan, qnan);
-            }
+        nan_outputs(s, fidx);
         if (s.berr)
             for (int g = tid; g < nrhs; g += kThreads) s.berr[(size_t)fidx * nrhs + g] = __builtin_inf();
         if (s.adj_out && tid == 0)
@@ -1674,10 +1721,12 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
                                              &anorm, be, &conv, L.red, fwd);
         if (refine && !conv) ok = false;
         if (it > it_max) it_max = it;
-        for (int e = tid; e < n * gc; e += kThreads) {
-            const int r = e % n, g = e / n;
-            st_z(X + r + (size_t)(g0 + g) * s.ldx, xs[e]);
-        }
+        if (X)
+            for (int e = tid; e < n * gc; e += kThreads) {
+                const int r = e % n, g = e / n;
+                st_z(X + r + (size_t)(g0 + g) * s.ldx, xs[e]);
+            }
+        if (s.C) project_cols(s, fidx, xs, g0, gc);
         if (s.berr && want_berr && tid == 0) {
 #pragma unroll
             for (int g = 0; g < kMaxGroup; ++g)
