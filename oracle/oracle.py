@@ -13,7 +13,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "libstc_oracle.so")
+# STC_ORACLE_LIB selects another build of the oracle (the sanitizer build)
+LIB = os.environ.get("STC_ORACLE_LIB") or os.path.join(HERE, "libstc_oracle.so")
 _lib = None
 
 

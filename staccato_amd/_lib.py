@@ -9,6 +9,9 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libstaccato_sweep.so")
+# the C++ mirror of the reference's sweep entry (KrylovROMSweep, IJCSACoupling);
+# STC_HOST_LIB selects another build of it (the sanitizer build of the CPU tests)
+HOST_LIB_PATH = os.environ.get("STC_HOST_LIB") or os.path.join(HERE, "lib", "libstaccato_host.so")
 
 STC_OK = 0
 STC_EINVAL = -1

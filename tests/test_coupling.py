@@ -388,7 +388,7 @@ def test_cpp_host_coupling_matches_oracle(mode):
     import ctypes
     from staccato_amd import _lib
     _lib.load()
-    host = ctypes.CDLL(_lib.LIB_PATH.replace("libstaccato_sweep.so", "libstaccato_host.so"))
+    host = ctypes.CDLL(_lib.HOST_LIB_PATH)
     subs, glob, gmaps, dRdY, dRdU, F = chain_case((20, 26, 18), 3, seed=41)
     freq = np.linspace(3.0, 850.0, 31)
     dims, Ks, Ms, Ds, Bs, Cs, us, fs, etas = [], [], [], [], [], [], [], [], []

@@ -280,7 +280,7 @@ def test_ld_and_empty_inputs(handle):
 def test_cpp_back_transform_kmor_writes_reference_format(handle, tmp_path):
     import ctypes
     from staccato_amd import _lib
-    host = ctypes.CDLL(_lib.LIB_PATH.replace("libstaccato_sweep.so", "libstaccato_host.so"))
+    host = ctypes.CDLL(_lib.HOST_LIB_PATH)
     f = host.stc_host_back_transform_kmor
     f.restype = ctypes.c_int
     rom, rng = synth.modal_rom(48, 13, 0.0, alpha=1e-4, beta=0.1)

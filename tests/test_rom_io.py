@@ -12,7 +12,7 @@ from staccato_amd import _lib
 @pytest.fixture(scope="module")
 def host():
     _lib.load()
-    h = ctypes.CDLL(_lib.LIB_PATH.replace("libstaccato_sweep.so", "libstaccato_host.so"))
+    h = ctypes.CDLL(_lib.HOST_LIB_PATH)
     h.stc_host_read_mtx_dense.argtypes = [ctypes.c_char_p, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                           ctypes.c_void_p, ctypes.c_longlong]
