@@ -29,7 +29,16 @@ CANDIDATES = [
     ("wide-2perCU-jb16", 2, 80 * 1024, 16),
     ("wide-1perCU", 2, 160 * 1024 - 256, 0),
     ("wide-1perCU-jb32", 2, 160 * 1024 - 256, 32),
+    # Gauss 3M complex products in the trailing update (complex64 only)
+    ("wide-3M-2perCU", 3, 80 * 1024, 0),
+    ("wide-3M-1perCU", 3, 160 * 1024 - 256, 0),
 ]
+
+
+def candidates_for(mode):
+    """The candidates that differ in `mode` (the 3M kernel exists for complex64 only)."""
+    complex64 = mode in ("refined", "fp32", 1)
+    return [c for c in CANDIDATES if complex64 or c[1] != 3]
 
 
 def default_cache_path():
@@ -102,7 +111,7 @@ def tune(handle, rom_id, freq_hz, F, mode, tol, sample=1024, reps=3, candidates=
     info_t = torch.empty(freq.size, dtype=torch.int32, device=dev)
     be_t = torch.empty((freq.size, nrhs), dtype=torch.float64, device=dev)
     results = []
-    for name, kernel, budget, jb in (candidates or CANDIDATES):
+    for name, kernel, budget, jb in (candidates or candidates_for(mode)):
         handle.set_policy(n, mode, kernel, budget, jb)
         with torch.cuda.stream(stream):
             handle.sweep_dev(rom_id, freq_t, F_t, X_t, info_t, be_t, mode=mode, tol=tol,

@@ -85,7 +85,8 @@ struct LaunchOrder {
 }  // namespace
 
 // How the blocked path of a launch runs (stc_policy): kernel 0 = automatic,
-// 1 = 16-column register panel, 2 = wide LDS panel; lds_budget bytes per
+// 1 = 16-column register panel, 2 = wide LDS panel, 3 = wide LDS panel with
+// Gauss 3M products in the trailing update (complex64; else as 2); lds_budget bytes per
 // workgroup (0 = automatic); jb_max caps the panel width (0 = none).
 struct Policy {
     int kernel = 0;
@@ -253,7 +254,7 @@ Policy policy_for(const stc_ctx* h, int n, int refine) {
 struct LaunchCfg {
     int refine = 0;
     int maxr = 1;
-    int wide = 0;   // sweep_kernel<., ., true>: wide LDS panels on the blocked path
+    int wide = 0;   // sweep_kernel<., ., wide>: 1 wide LDS panels on the blocked path, 2 with 3M products
     int lds = 0;
     int grid = 1;
     size_t slot_elems = 0;
@@ -282,12 +283,13 @@ int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine
     // automatic: systems that need a factor slot go to the wide-panel kernel
     // when the largest order exceeds 128 (below, the register panel at three
     // workgroups per CU measured faster), the kernel is built for these rows
-    // per lane and every system's panel fits its LDS; policy kernel 2 takes it
-    // for every launch it fits
+    // per lane and every system's panel fits its LDS; policy kernels 2 and 3
+    // take it for every launch it fits (3: Gauss 3M products, complex64 only)
     bool blocked = false;
     for (int i = 0; i < nsys; ++i) blocked = blocked || !sys[i].lay_small;
     cfg->wide = 0;
-    if (((pol.kernel == 0 && blocked && maxn > 128) || pol.kernel == 2) && sweep_kernel_fn(refine, maxr, 1)) {
+    const int wkind = (pol.kernel == 3 && sweep_kernel_fn(refine, maxr, 2)) ? 2 : 1;
+    if (((pol.kernel == 0 && blocked && maxn > 128) || pol.kernel >= 2) && sweep_kernel_fn(refine, maxr, wkind)) {
         int wcap = wide_jb_max(ts);
         if (pol.jb_max > 0) wcap = std::max(kSB, std::min(pol.jb_max, wcap));
         // automatic LDS budget (measured on MI355X, n = 160 .. 1024): two
@@ -297,7 +299,7 @@ int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine
         const int budget = pol.lds_budget > 0 ? pol.lds_budget : auto_budget;
         int lds = 0;
         if (layout_wide(sys, nsys, ts, adj, wcap, maxr, budget, &lds)) {  // writes only when all fit
-            cfg->wide = 1;
+            cfg->wide = wkind;
             cfg->lds = lds;
         }
     }
@@ -525,7 +527,7 @@ int stc_set_policy(stc_handle h, int n, int mode, const stc_policy* p) {
         h->policy.erase(key);
         return STC_OK;
     }
-    if (p->kernel < 0 || p->kernel > 2 || p->jb_max < 0 ||
+    if (p->kernel < 0 || p->kernel > 3 || p->jb_max < 0 ||
         (p->lds_budget != 0 && (p->lds_budget < 16384 || p->lds_budget > kBudgetLarge)))
         return STC_EINVAL;
     Policy q;

@@ -13,7 +13,7 @@
 #error "STC_INST_MAXR (1, 2, 4, 8 or 16) must be defined"
 #endif
 #ifndef STC_INST_WIDE
-#error "STC_INST_WIDE (0 or 1) must be defined"
+#error "STC_INST_WIDE (0: register panel, 1: wide panel, 2: wide panel with 3M products) must be defined"
 #endif
 
 #define STC_CAT2(a, b, c, d) a##b##_##c##_##d
@@ -25,10 +25,10 @@ namespace stc {
 
 using InstT = typename std::conditional<STC_INST_REFINE != 0, float, double>::type;
 
-void* STC_FN_PTR() { return reinterpret_cast<void*>(&sweep_kernel<InstT, STC_INST_MAXR, STC_INST_WIDE != 0>); }
+void* STC_FN_PTR() { return reinterpret_cast<void*>(&sweep_kernel<InstT, STC_INST_MAXR, STC_INST_WIDE>); }
 
 hipError_t STC_FN_LAUNCH(int grid, int lds, hipStream_t st, const KParams& kp) {
-    hipLaunchKernelGGL((sweep_kernel<InstT, STC_INST_MAXR, STC_INST_WIDE != 0>), dim3(grid), dim3(kThreads), lds,
+    hipLaunchKernelGGL((sweep_kernel<InstT, STC_INST_MAXR, STC_INST_WIDE>), dim3(grid), dim3(kThreads), lds,
                        st, kp);
     return hipGetLastError();
 }

@@ -1595,10 +1595,11 @@ __device__ __forceinline__ void nan_outputs(const SysDesc& s, int fidx) {
 }
 
 // ---- one point ---------------------------------------------------------------------------
-// WIDE: blocked systems use the wide-panel factorisation (factor_wide, LDS
-// panel of lay.jb columns, 16 x 16 inverted diagonal blocks); otherwise the
-// register panel (factor_blocked, lay.jb x lay.jb blocks).
-template <typename T, int MAXR, bool WIDE>
+// WIDE != 0: blocked systems use the wide-panel factorisation (factor_wide, LDS
+// panel of lay.jb columns, 16 x 16 inverted diagonal blocks), WIDE == 2 with
+// Gauss 3M products in its trailing update; WIDE == 0: the register panel
+// (factor_blocked, lay.jb x lay.jb blocks).
+template <typename T, int MAXR, int WIDE>
 __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const KParams& kp, char* smem) {
     const int n = s.n, tid = threadIdx.x, nrhs = s.nrhs;
     const double w = s.freq ? kTwoPi * s.freq[fidx] : 0.0, w2 = w * w;
@@ -1640,7 +1641,7 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
         // right-hand sides eliminated with the factorisation (the first group of
         // a forward sweep)
         const int nr = (kp.flags & (kAdjoint | kFactorOut)) ? 0 : min(nrhs, lay.G);
-        if (WIDE) factor_wide<T, MAXR>(s, w, w2, S, lay, L, nr);
+        if (WIDE) factor_wide<T, MAXR, WIDE == 2>(s, w, w2, S, lay, L, nr);
         else factor_blocked<T, MAXR>(s, w, w2, S, lay, L, nr);
         fused = nr;
         LU = S;
@@ -1759,7 +1760,7 @@ __device__ __forceinline__ SysDesc load_uniform(const SysDesc* p) {
 // file is split that many ways (at most 168 registers per wavefront for three,
 // 256 for two).  The wide-panel kernel (WIDE) runs one or two per CU (its LDS
 // panel decides) with up to 256 registers.
-template <typename T, int MAXR, bool WIDE>
+template <typename T, int MAXR, int WIDE>
 __global__ void __launch_bounds__(kThreads)
     __attribute__((amdgpu_waves_per_eu(WIDE ? 2 : wg_per_cu(sizeof(cx<T>)), WIDE ? 2 : wg_per_cu(sizeof(cx<T>)))))
 sweep_kernel(KParams kp) {

@@ -51,6 +51,19 @@ __device__ __forceinline__ void cmfma(typename Mfma<T>::acc& dre, typename Mfma<
     dim = M::run(a.im, bre, dim);
 }
 
+// Gauss (3M) form of d += a * b over three accumulators: x += a_r b_r,
+// y += a_i b_i, z += (a_r + a_i)(b_r + b_i); with x, y, z started at (d_r, 0,
+// d_r + d_i) the result is d_r = x - y, d_i = z - x - y.  Three real products
+// instead of four.
+template <typename T>
+__device__ __forceinline__ void cmfma3(typename Mfma<T>::acc& x, typename Mfma<T>::acc& y,
+                                       typename Mfma<T>::acc& z, cx<T> a, T asum, T bre, T bim) {
+    using M = Mfma<T>;
+    x = M::run(a.re, bre, x);
+    y = M::run(a.im, bim, y);
+    z = M::run(asum, bre + bim, z);
+}
+
 template <typename T>
 __device__ __forceinline__ void acc_zero(typename Mfma<T>::acc& a) {
 #pragma unroll
@@ -391,7 +404,9 @@ __device__ __forceinline__ void inpanel_update(cx<T>* __restrict__ P, int m, int
 // logical order); Linv: its inverted 16 x 16 diagonal blocks (ld 16).
 // ASM: first panel, whose trailing entries are assembled from the operators
 // (physical row r, column c of A(w) is element r + c n) instead of loaded.
-template <typename T, bool ASM>
+// G3: the C update uses the Gauss 3M product (cmfma3) instead of four real
+// products per complex one.
+template <typename T, bool ASM, bool G3>
 __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, double w2, cx<T>* __restrict__ S, int ld,
                                               const int* rmap, int j0, int jbl, int n, int ncol, const cx<T>* P,
                                               int ldp, const cx<T>* Linv) {
@@ -484,6 +499,7 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
             }
         };
         acc_t cre[TA], cim[TA];
+        acc_t cy[G3 ? TA : 1];  // G3: sum of a_i b_i
         int prow[TA];
         load_c(0, cre, cim, prow);
         for (int r0 = 0; r0 < m2; r0 += kSB * TA) {
@@ -491,6 +507,14 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
             int nprow[TA];
             const bool more = r0 + kSB * TA < m2;
             if (more && !ASM) load_c(r0 + kSB * TA, nre, nim, nprow);
+            if (G3) {
+#pragma unroll
+                for (int a = 0; a < TA; ++a) {
+                    acc_zero<T>(cy[a]);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) cim[a][r] += cre[a][r];
+                }
+            }
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 if (b < nbl) {
@@ -506,10 +530,27 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
                             const int row = min(r0 + kSB * a + li, m2 - 1);
                             lv[a] = ldc<3>(P + ((jbl + row) + (size_t)k * ldp));
                         }
+                        if (G3) {
+                            const T us = -(ure[b][s] + uim[b][s]);
 #pragma unroll
-                        for (int a = 0; a < TA; ++a) cmfma<T>(cre[a], cim[a], ua, lv[a].re, lv[a].im);
+                            for (int a = 0; a < TA; ++a)
+                                cmfma3<T>(cre[a], cy[a], cim[a], ua, us, lv[a].re, lv[a].im);
+                        } else {
+#pragma unroll
+                            for (int a = 0; a < TA; ++a) cmfma<T>(cre[a], cim[a], ua, lv[a].re, lv[a].im);
+                        }
                     }
                 }
+            }
+            if (G3) {
+#pragma unroll
+                for (int a = 0; a < TA; ++a)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const T x = cre[a][r], y = cy[a][r];
+                        cre[a][r] = x - y;
+                        cim[a][r] = cim[a][r] - x - y;
+                    }
             }
 #pragma unroll
             for (int a = 0; a < TA; ++a) {
@@ -540,7 +581,7 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
 }
 
 // ---- the wide-panel factorisation ------------------------------------------------------------
-template <typename T, int MAXR>
+template <typename T, int MAXR, bool G3>
 __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w2, cx<T>* __restrict__ S,
                                             const Layout& lay, const Lds& L, int nr) {
     const int n = s.n, tid = threadIdx.x;
@@ -609,8 +650,8 @@ __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w
             for (int c = 0; c < jbl; ++c) stc<1>(S + (pr + (size_t)(j0 + c) * n), ldc<3>(P + (r + (size_t)c * ldp)));
         }
         // ---- trailing columns
-        if (j0 == 0) wide_trailing<T, true>(s, w, w2, S, n, L.rmap, j0, jbl, n, n + nr, P, ldp, Linv);
-        else wide_trailing<T, false>(s, w, w2, S, n, L.rmap, j0, jbl, n, n + nr, P, ldp, Linv);
+        if (j0 == 0) wide_trailing<T, true, G3>(s, w, w2, S, n, L.rmap, j0, jbl, n, n + nr, P, ldp, Linv);
+        else wide_trailing<T, false, G3>(s, w, w2, S, n, L.rmap, j0, jbl, n, n + nr, P, ldp, Linv);
         __syncthreads();
     }
     if (sizeof(T) == 4) {
