@@ -160,6 +160,11 @@ template <typename T>
 __device__ __forceinline__ cx<T> cbcast(cx<T> z, int l) {
     return cmk<T>(lane_read(z.re, l), lane_read(z.im, l));
 }
+// z from lane src of the wavefront (one ds_bpermute per part).
+template <typename T>
+__device__ __forceinline__ cx<T> cshfl(cx<T> z, int src) {
+    return cmk<T>(__shfl(z.re, src, 64), __shfl(z.im, src, 64));
+}
 template <typename T>
 __device__ __forceinline__ cx<T> to_t(z64 a) { return cmk<T>((T)a.re, (T)a.im); }
 template <typename T>
@@ -597,38 +602,45 @@ __device__ __forceinline__ void invert_diag(const cx<T>* P, int ldp, int jb, cx<
 // Only the refined mode's block solves use them.  Wave w inverts blocks w,
 // w+4, ... (lanes = rows, bs <= 16) from the final factors in the slot, so no
 // block barrier is needed; the result goes to the block store (ld = block width).
+// Inverses of the bs x bs (bs <= 16) diagonal blocks of U (logical rows of the
+// slot): block b to Ug + b bs^2, kb x kb column-major (ld kb).  Back
+// substitution X = U^-1, one block per wavefront at a time: lane (i, g) =
+// (lane % 16, lane / 16) holds X[i][4g .. 4g+3]; step k scales row k by
+// 1 / U[k][k] and the rows above take row k from lane 16 g + k (one cross-lane
+// read per complex entry, the four column groups in parallel).
 template <typename T>
 __device__ __forceinline__ void uinv_blocks(const cx<T>* __restrict__ S, int n, const int* rmap, int bs,
                                             cx<T>* __restrict__ Ug) {
     constexpr int B = 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 15, g = lane >> 4;
     const int nb = (n + bs - 1) / bs;
     for (int b = wave; b < nb; b += kWaves) {
         const int k0 = b * bs, kb = min(bs, n - k0);
-        const int pr = lds1<3>(rmap + k0 + min(lane, kb - 1));
-        cx<T> u[B], x[B];
+        // rows beyond kb read row kb - 1 and are never stored or read by other rows
+        const int pr = lds1<3>(rmap + k0 + min(i, kb - 1));
+        cx<T> u[B], x[4];
 #pragma unroll
-        for (int c = 0; c < B; ++c) {
-            u[c] = ldc<1>(S + (pr + (k0 + min(c, kb - 1)) * n));
-            x[c] = lane == c ? cmk<T>(T(1), T(0)) : czero<T>();
-        }
+        for (int k = 0; k < B; ++k) u[k] = ldc<1>(S + (pr + (size_t)(k0 + min(k, kb - 1)) * n));
+        const cx<T> rd = crecip(ldc<1>(S + (pr + (size_t)(k0 + min(i, kb - 1)) * n)));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[q] = i == 4 * g + q ? cmk<T>(T(1), T(0)) : czero<T>();
 #pragma unroll
         for (int k = B - 1; k >= 0; --k) {
             if (k < kb) {
-                const cx<T> rk = crecip(cbcast(u[k], k));
-                const cx<T> uk = lane < k ? u[k] : czero<T>();
 #pragma unroll
-                for (int c = 0; c < B; ++c) {
-                    if (c < kb) {
-                        if (lane == k) x[c] = cmul(x[c], rk);
-                        cfms(x[c], uk, cbcast(x[c], k));
-                    }
+                for (int q = 0; q < 4; ++q) {
+                    if (i == k) x[q] = cmul(x[q], rd);
+                    const cx<T> xk = cshfl(x[q], 16 * g + k);
+                    if (i < k) cfms(x[q], u[k], xk);
                 }
             }
         }
 #pragma unroll
-        for (int c = 0; c < B; ++c)
-            if (c < kb && lane < kb) stc<1>(Ug + ((size_t)b * bs * bs + lane + c * kb), x[c]);
+        for (int q = 0; q < 4; ++q) {
+            const int c = 4 * g + q;
+            if (c < kb && i < kb) stc<1>(Ug + ((size_t)b * bs * bs + i + c * kb), x[q]);
+        }
     }
 }
 

@@ -306,32 +306,32 @@ __device__ __forceinline__ void subpanel_factor(cx<T>* __restrict__ P, int m, in
 }
 
 // ---- inverse of a unit-lower 16 x 16 diagonal block ----------------------------------------
-// From Q (LDS, ld ldq; kb <= 16 valid rows/columns).  Wave w computes
-// columns w, w+4, ... (lanes = rows).  Linv (LDS, 16 x 16, ld 16) is zero
-// outside the kb x kb block; Lg (slot, ld kb) gets the kb x kb block for the
-// triangular solves.
+// From Q (LDS, ld ldq; kb <= 16 valid rows/columns), by forward substitution
+// X[i][c] -= L[i][k] X[k][c] (i > k) over all 256 threads, one entry each;
+// X[k][c] comes from the same wavefront by one cross-lane read per part.
+// Linv (LDS, 16 x 16, ld 16) is zero outside the kb x kb block; Lg (slot,
+// ld kb) gets the kb x kb block for the triangular solves.
 template <typename T>
 __device__ __forceinline__ void invert_unit_lower16(const cx<T>* Q, int ldq, int kb, cx<T>* Linv,
                                                     cx<T>* __restrict__ Lg) {
-    constexpr int CPW = kSB / kWaves;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    cx<T> xl[CPW];
+    static_assert(kThreads == kSB * kSB, "one thread per entry of the block");
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int i = tid & 15, c = tid >> 4;  // thread (i, c) holds X[i][c]
+    cx<T> lrow[kSB - 1];                   // L[i][k] for k < i (zero for rows >= kb)
 #pragma unroll
-    for (int q = 0; q < CPW; ++q) {
-        const int j = wave + kWaves * q;
-        xl[q] = (lane == j && j < kb) ? cmk<T>(T(1), T(0)) : czero<T>();
-    }
-    for (int k = 0; k < kb - 1; ++k) {
-        const cx<T> lk = ldz<3>(Q + (min(lane, kb - 1) + (size_t)k * ldq), lane > k && lane < kb);
+    for (int k = 0; k < kSB - 1; ++k)
+        lrow[k] = ldz<3>(Q + (min(i, kb - 1) + (size_t)min(k, kb - 1) * ldq), k < i && i < kb);
+    cx<T> x = (i == c && c < kb) ? cmk<T>(T(1), T(0)) : czero<T>();
 #pragma unroll
-        for (int q = 0; q < CPW; ++q) cfms(xl[q], lk, cbcast(xl[q], k));
+    for (int k = 0; k < kSB - 1; ++k) {
+        if (k < kb - 1) {
+            // X[k][c] sits on lane k + 16 (c % 4) of this wavefront
+            const cx<T> xk = cshfl(x, (lane & 48) | k);
+            if (i > k) cfms(x, lrow[k], xk);
+        }
     }
-#pragma unroll
-    for (int q = 0; q < CPW; ++q) {
-        const int j = wave + kWaves * q;
-        if (lane < kSB) stc<3>(Linv + (lane + j * kSB), xl[q]);
-        if (j < kb && lane < kb) stc<1>(Lg + (lane + (size_t)j * kb), xl[q]);
-    }
+    stc<3>(Linv + (i + c * kSB), x);
+    if (i < kb && c < kb) stc<1>(Lg + (i + (size_t)c * kb), x);
 }
 
 // ---- update of the panel's columns right of sub-panel c0 (MFMA, LDS) ------------------------
