@@ -25,6 +25,11 @@ ARCH = "gfx950"
 # (name, kernel, lds_budget bytes, jb_max)
 CANDIDATES = [
     ("register-panel", 1, 0, 0),
+    # the register-panel kernel at two workgroups per CU (80 KB of LDS each):
+    # systems whose whole matrix fits then stay LDS-resident (complex64 n <= 96).
+    # One workgroup per CU (n = 128 complex64 whole in LDS) measured 4x slower
+    # than three register-panel workgroups per CU and is not a candidate.
+    ("lds-2perCU", 1, 80 * 1024, 0),
     ("wide-2perCU", 2, 80 * 1024, 0),
     ("wide-2perCU-jb16", 2, 80 * 1024, 16),
     ("wide-1perCU", 2, 160 * 1024 - 256, 0),

@@ -11,14 +11,18 @@
 //      |re| + |im|, full-row interchanges, unit-lower L and U in place,
 //      - small n: the whole matrix lives in LDS, unblocked right-looking;
 //      - larger n: the matrix lives in a per-workgroup slot of global memory
-//        (cache resident) and is factored blocked right-looking: an LDS panel
-//        of jb columns, deferred row interchanges, U12 = L11^{-1} A12 and the
-//        trailing update A22 -= L21 U12 per chunk of columns,
+//        (reused point after point; its traffic goes through L2 and, once the
+//        resident slots exceed the caches, HBM) and is factored blocked
+//        right-looking: a panel of jb columns (registers, or LDS in the wide
+//        path of stc_wide.h), deferred row interchanges, U12 = L11^{-1} A12
+//        and the trailing update A22 -= L21 U12 per chunk of columns,
 //   3. solves for all right-hand sides (KrylovROMSubstructure.cpp:1430),
 //   4. optionally refines in FP64: r = F - A x with A re-assembled in FP64,
 //      A d = r with the T factors, x += d, until the normwise backward error
 //      ||r||_inf / (||A||_inf ||x||_inf + ||F||_inf) <= tol,
-//   5. optionally runs the adjoint with the same factors.
+//   5. optionally runs the adjoint with the same factors,
+//   6. optionally projects the solution, Y = C X (KrylovROMSubstructure.cpp:1447-1451),
+//      so that only Y leaves the workgroup.
 // A(w) is never written to global memory on the small path; on the blocked
 // path the only copy is the workgroup's factor slot.
 #pragma once

@@ -1,26 +1,29 @@
 // stc_wide.h -- wide-panel blocked LU of the sweep kernel (gfx950).
 //
-// The blocked path of sweep_kernel<T, MAXR, true>.  It factors A(w) in the
-// workgroup's global factor slot, right-looking, with panels of jb = 16..64
-// columns held in LDS (one or two workgroups per CU, so the resident slots
-// stay within the on-chip caches), and cuts the trailing-matrix traffic by
-// jb / 16 against a 16-column panel:
+// The blocked path of sweep_kernel<T, MAXR, 1> (and <T, MAXR, 2>: Gauss 3M
+// products in the trailing update).  It factors A(w) in the workgroup's
+// global factor slot, right-looking, with panels of jb = 16..64 columns held
+// in LDS (one or two workgroups per CU), and cuts the trailing-matrix traffic
+// by jb / 16 against a 16-column panel:
 //
-//   panel (m = n - j0 rows x jb columns, LDS, logical row order):
-//     per 16-column sub-panel: pivoted LU in registers (wave w owns columns
-//     w, w+4, w+8, w+12; lane l owns rows l + 64 t), rows permuted in the LDS
-//     panel afterwards; the 16 x 16 unit-lower diagonal block inverted; the
-//     panel's remaining columns updated on MFMA (U rows = Linv A, rows below
-//     -= L21 U);
+//   panel (m = n - j0 rows x jb columns, LDS, logical row order, assembled
+//   from the operators at j0 = 0):
+//     per 16-column sub-panel: pivoted LU with the rows distributed over the
+//     threads (thread t holds rows t + 256 u in registers, one barrier per
+//     column), rows permuted in the LDS panel afterwards; the 16 x 16
+//     unit-lower diagonal block inverted (one entry per thread); the panel's
+//     remaining columns updated on MFMA (U rows = Linv A, rows below -= L21 U);
 //   trailing columns (matrix, then the first right-hand sides), 16 per chunk,
 //     one wavefront per chunk, no block barrier:
 //     U12 = L11^{-1} A12 by block forward substitution on MFMA in registers
 //     (16-row blocks: acc_b -= L_ba U_a, U_b = Linv_b acc_b), U12 to the slot,
 //     then C -= L21 U12 over tiles of 16 TA rows, the next tile's C loaded
-//     before the current tile's products.
+//     before the current tile's products (assembled from the operators at the
+//     first panel).
 //
 // Rows never move in the slot (row map, as in the 16-column path); only the
-// LDS panel is kept in logical order.  Complex products take four real MFMAs.
+// LDS panel is kept in logical order.  Complex products take four real MFMAs
+// (three in the Gauss form).
 //
 // MFMA operand layout used throughout (16x16x4, one wavefront; li = lane % 16,
 // lg = lane / 16): lane supplies A[li][k-slot lg] and B[k-slot lg][li]; the
