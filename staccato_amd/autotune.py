@@ -34,6 +34,10 @@ CANDIDATES = [
     ("wide-2perCU-jb16", 2, 80 * 1024, 16),
     ("wide-1perCU", 2, 160 * 1024 - 256, 0),
     ("wide-1perCU-jb32", 2, 160 * 1024 - 256, 32),
+    # two-level update: several LDS panels per 64-column step (32 complex128),
+    # one deferred update of the columns right of the step
+    ("wide-2level-2perCU", 4, 80 * 1024, 0),
+    ("wide-2level-1perCU", 4, 160 * 1024 - 256, 0),
     # Gauss 3M complex products in the trailing update (complex64 only)
     ("wide-3M-2perCU", 3, 80 * 1024, 0),
     ("wide-3M-1perCU", 3, 160 * 1024 - 256, 0),

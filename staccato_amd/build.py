@@ -21,9 +21,10 @@ HOST_OUT = os.path.join(OUT_DIR, "libstaccato_host.so")
 
 # (refine, rows per lane, panel kind) triples of sweep_kernel, one object each
 # (panel kind 0: register panel, 1: wide LDS panel, 2: wide panel with Gauss 3M
-# products, complex64 only); the list matches STC_INSTANCES in csrc/stc_dispatch.hip
+# products, complex64 only, 3: wide panel with the two-level update); the list matches STC_INSTANCES in csrc/stc_dispatch.hip
 INSTANCES = ([(r, m, 0) for r in (0, 1) for m in (1, 2, 4, 8, 16)] +
-             [(0, m, 1) for m in (2, 4, 8)] + [(1, m, w) for w in (1, 2) for m in (2, 4, 8, 16)])
+             [(0, m, w) for w in (1, 3) for m in (2, 4, 8)] +
+             [(1, m, w) for w in (1, 2, 3) for m in (2, 4, 8, 16)])
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -86,7 +86,9 @@ struct LaunchOrder {
 
 // How the blocked path of a launch runs (stc_policy): kernel 0 = automatic,
 // 1 = 16-column register panel, 2 = wide LDS panel, 3 = wide LDS panel with
-// Gauss 3M products in the trailing update (complex64; else as 2); lds_budget bytes per
+// Gauss 3M products in the trailing update (complex64; else as 2), 4 = wide LDS
+// panel with the two-level update (several LDS panels per step, one deferred
+// rank-64 / rank-32 update of the columns right of the step); lds_budget bytes per
 // workgroup (0 = automatic); jb_max caps the panel width (0 = none).
 struct Policy {
     int kernel = 0;
@@ -176,8 +178,10 @@ constexpr int kBudgetLarge = 160 * 1024 - 256;  // one workgroup per CU
 // Pick the LDS budget of a launch (forced_budget > 0: that one) and fill each
 // system's layout for the register-panel kernel: the largest number of
 // workgroups per CU the kernel is built for at which every blocked system
-// still gets a panel of at least 12 columns (a thin panel starves the
-// trailing update).
+// still gets a panel of at least 12 columns, or as many as the registers
+// hold when that is fewer (a thin panel starves the trailing update; at
+// n = 1024 complex128 the registers cap it at 4, and two workgroups per CU
+// measured 1126 against 857 points/s for one).
 int plan_systems(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, int forced_budget,
                  int* lds_out) {
     int budget = kBudgetLarge;
@@ -191,7 +195,7 @@ int plan_systems(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, 
             for (int i = 0; i < nsys; ++i) {
                 const SysDesc& s = sys[i];
                 Layout L = plan_layout(s.n, s.nrhs, ts, b, jb_cap, adj, maxr);
-                if (L.lds > b || (!L.small && L.jb < 12)) ok = false;
+                if (L.lds > b || (!L.small && L.jb < std::min(12, jb_cap))) ok = false;
             }
             if (ok) {
                 budget = b;
@@ -217,7 +221,7 @@ int plan_systems(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, 
 // Wide-panel layout of every system within `budget` bytes of LDS; false when
 // one of them does not fit.
 bool layout_wide(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, int budget,
-                 int* lds_out) {
+                 bool two_level, int* lds_out) {
     int lds = 0;
     for (int i = 0; i < nsys; ++i) {
         const Layout L = plan_layout_wide(sys[i].n, sys[i].nrhs, ts, budget, jb_cap, adj, maxr);
@@ -230,6 +234,10 @@ bool layout_wide(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, 
         sys[i].lay_jb = L.jb;
         sys[i].lay_cw = L.cw;
         sys[i].lay_G = L.G;
+        // two-level update: steps of the widest panel the trailing update holds
+        // (64 columns complex64, 32 complex128), a multiple of the LDS panel
+        const int jmax = wide_jb_max(ts);
+        sys[i].lay_jo = (two_level && !L.small && L.jb >= kSB && L.jb < jmax) ? (jmax / L.jb) * L.jb : 0;
     }
     *lds_out = lds;
     return true;
@@ -254,7 +262,7 @@ Policy policy_for(const stc_ctx* h, int n, int refine) {
 struct LaunchCfg {
     int refine = 0;
     int maxr = 1;
-    int wide = 0;   // sweep_kernel<., ., wide>: 1 wide LDS panels on the blocked path, 2 with 3M products
+    int wide = 0;   // sweep_kernel<., ., wide>: 1 wide LDS panels on the blocked path, 2 with 3M products, 3 two-level
     int lds = 0;
     int grid = 1;
     size_t slot_elems = 0;
@@ -288,7 +296,8 @@ int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine
     bool blocked = false;
     for (int i = 0; i < nsys; ++i) blocked = blocked || !sys[i].lay_small;
     cfg->wide = 0;
-    const int wkind = (pol.kernel == 3 && sweep_kernel_fn(refine, maxr, 2)) ? 2 : 1;
+    const int wkind = (pol.kernel == 3 && sweep_kernel_fn(refine, maxr, 2)) ? 2
+                      : (pol.kernel == 4 && sweep_kernel_fn(refine, maxr, 3)) ? 3 : 1;
     if (((pol.kernel == 0 && blocked && maxn > 128) || pol.kernel >= 2) && sweep_kernel_fn(refine, maxr, wkind)) {
         int wcap = wide_jb_max(ts);
         if (pol.jb_max > 0) wcap = std::max(kSB, std::min(pol.jb_max, wcap));
@@ -298,7 +307,7 @@ int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine
         const int auto_budget = maxn > (ts == 8 ? 512 : 256) ? kBudgetLarge : kBudgetSmall;
         const int budget = pol.lds_budget > 0 ? pol.lds_budget : auto_budget;
         int lds = 0;
-        if (layout_wide(sys, nsys, ts, adj, wcap, maxr, budget, &lds)) {  // writes only when all fit
+        if (layout_wide(sys, nsys, ts, adj, wcap, maxr, budget, wkind == 3, &lds)) {  // writes only when all fit
             cfg->wide = wkind;
             cfg->lds = lds;
         }
@@ -527,7 +536,7 @@ int stc_set_policy(stc_handle h, int n, int mode, const stc_policy* p) {
         h->policy.erase(key);
         return STC_OK;
     }
-    if (p->kernel < 0 || p->kernel > 3 || p->jb_max < 0 ||
+    if (p->kernel < 0 || p->kernel > 4 || p->jb_max < 0 ||
         (p->lds_budget != 0 && (p->lds_budget < 16384 || p->lds_budget > kBudgetLarge)))
         return STC_EINVAL;
     Policy q;

@@ -7,7 +7,7 @@ namespace stc {
 
 // (refine, rows per lane, panel kind) triples that are built (panel kind 0:
 // register panel, 1: wide LDS panel, 2: wide panel with Gauss 3M products,
-// complex64 only); the wide-panel kernel is not built where its LDS panel
+// complex64 only, 3: wide panel with the two-level update); the wide-panel kernel is not built where its LDS panel
 // cannot hold 16 columns (complex128 beyond n = 512) or where every system
 // fits the LDS whole (n <= 64).
 #define STC_INSTANCES(X)                                                         \
@@ -15,7 +15,9 @@ namespace stc {
     X(1, 1, 0) X(1, 2, 0) X(1, 4, 0) X(1, 8, 0) X(1, 16, 0)                      \
     X(0, 2, 1) X(0, 4, 1) X(0, 8, 1)                                             \
     X(1, 2, 1) X(1, 4, 1) X(1, 8, 1) X(1, 16, 1)                                \
-    X(1, 2, 2) X(1, 4, 2) X(1, 8, 2) X(1, 16, 2)
+    X(1, 2, 2) X(1, 4, 2) X(1, 8, 2) X(1, 16, 2)                                \
+    X(0, 2, 3) X(0, 4, 3) X(0, 8, 3)                                             \
+    X(1, 2, 3) X(1, 4, 3) X(1, 8, 3) X(1, 16, 3)
 
 #define STC_DECL(R, M, W)                                                        \
     void* sweep_kernel_fn_##R##_##M##_##W();                                     \

@@ -70,6 +70,8 @@ struct SysDesc {
     int lay_jb;
     int lay_cw;
     int lay_G;
+    int lay_jo;          // wide path: outer width of the two-level update (<= lay_jb: none)
+    int pad2_;
 };
 
 struct KParams {
@@ -94,6 +96,7 @@ struct Layout {
     int cw;         // trailing column chunk width (blocked)
     int G;          // right-hand sides per solve group
     int lds;        // bytes needed
+    int jo = 0;     // wide path: columns per two-level step (<= jb: one level)
 };
 
 __host__ __device__ inline int align16(long long b) { return (int)((b + 15) & ~15LL); }

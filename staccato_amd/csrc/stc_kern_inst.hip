@@ -13,7 +13,7 @@
 #error "STC_INST_MAXR (1, 2, 4, 8 or 16) must be defined"
 #endif
 #ifndef STC_INST_WIDE
-#error "STC_INST_WIDE (0: register panel, 1: wide panel, 2: wide panel with 3M products) must be defined"
+#error "STC_INST_WIDE (0: register panel, 1: wide panel, 2: wide panel with 3M products, 3: wide panel with the two-level update) must be defined"
 #endif
 
 #define STC_CAT2(a, b, c, d) a##b##_##c##_##d

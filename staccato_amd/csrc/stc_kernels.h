@@ -1613,7 +1613,8 @@ __device__ __forceinline__ void nan_outputs(const SysDesc& s, int fidx) {
 // ---- one point ---------------------------------------------------------------------------
 // WIDE != 0: blocked systems use the wide-panel factorisation (factor_wide, LDS
 // panel of lay.jb columns, 16 x 16 inverted diagonal blocks), WIDE == 2 with
-// Gauss 3M products in its trailing update; WIDE == 0: the register panel
+// Gauss 3M products in its trailing update, WIDE == 3 with the two-level
+// update (lay.jo columns per step); WIDE == 0: the register panel
 // (factor_blocked, lay.jb x lay.jb blocks).
 template <typename T, int MAXR, int WIDE>
 __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const KParams& kp, char* smem) {
@@ -1625,6 +1626,7 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
     lay.cw = s.lay_cw;
     lay.G = s.lay_G;
     lay.lds = 0;
+    lay.jo = s.lay_jo;
     const Lds L = carve(smem, n);
     const bool refine = (kp.flags & kRefine) != 0;
     const bool want_berr = refine || (kp.flags & kWantBerr) || (s.berr != nullptr);
@@ -1643,7 +1645,9 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
         LU = A;
         solvep = L.main + align16((long long)nn * sizeof(cx<T>));
     } else {
-        const bool bad_jb = WIDE ? (lay.jb < kSB || lay.jb % kSB != 0 || lay.jb > kSB * WideCfg<T>::NB)
+        const bool bad_jb = WIDE ? (lay.jb < kSB || lay.jb % kSB != 0 || lay.jb > kSB * WideCfg<T>::NB ||
+                                    (WIDE == 3 && lay.jo > lay.jb &&
+                                     (lay.jo % lay.jb != 0 || lay.jo > kSB * WideCfg<T>::NB)))
                                  : (lay.jb > kWaves * panel_cols_per_wave((int)sizeof(cx<T>), MAXR) || lay.jb > 32);
         if (kp.scratch == nullptr || kp.slot_elems < slot_elems_for(n, WIDE ? kSB : lay.jb) || bad_jb) {
             // no factor slot, or a panel wider than the register panel (the host
@@ -1657,7 +1661,7 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
         // right-hand sides eliminated with the factorisation (the first group of
         // a forward sweep)
         const int nr = (kp.flags & (kAdjoint | kFactorOut)) ? 0 : min(nrhs, lay.G);
-        if (WIDE) factor_wide<T, MAXR, WIDE == 2>(s, w, w2, S, lay, L, nr);
+        if (WIDE) factor_wide<T, MAXR, WIDE == 2, WIDE == 3>(s, w, w2, S, lay, L, nr);
         else factor_blocked<T, MAXR>(s, w, w2, S, lay, L, nr);
         fused = nr;
         LU = S;

@@ -1,7 +1,7 @@
 // stc_wide.h -- wide-panel blocked LU of the sweep kernel (gfx950).
 //
-// The blocked path of sweep_kernel<T, MAXR, 1> (and <T, MAXR, 2>: Gauss 3M
-// products in the trailing update).  It factors A(w) in the workgroup's
+// The blocked path of sweep_kernel<T, MAXR, 1> (<T, MAXR, 2>: Gauss 3M
+// products in the trailing update; <T, MAXR, 3>: two-level update).  It factors A(w) in the workgroup's
 // global factor slot, right-looking, with panels of jb = 16..64 columns held
 // in LDS (one or two workgroups per CU), and cuts the trailing-matrix traffic
 // by jb / 16 against a 16-column panel:
@@ -409,7 +409,12 @@ __device__ __forceinline__ void inpanel_update(cx<T>* __restrict__ P, int m, int
 // (physical row r, column c of A(w) is element r + c n) instead of loaded.
 // G3: the C update uses the Gauss 3M product (cmfma3) instead of four real
 // products per complex one.
-template <typename T, bool ASM, bool G3>
+// LG: the factored columns [j0, j0 + jbl) are read from the slot (logical row
+// r at physical row rmap[r]) instead of the LDS panel, and Linv points to
+// their inverted diagonal blocks in the slot: the deferred update of a
+// two-level step, whose jbl = 16..64 columns were factored as several LDS
+// panels (every 16-block full).
+template <typename T, bool ASM, bool G3, bool LG>
 __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, double w2, cx<T>* __restrict__ S, int ld,
                                               const int* rmap, int j0, int jbl, int n, int ncol, const cx<T>* P,
                                               int ldp, const cx<T>* Linv) {
@@ -422,6 +427,11 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
     const int c0 = j0 + jbl, m2 = n - c0;
     const int nbl = (jbl + kSB - 1) / kSB;
     const int nch = (ncol - c0 + kSB - 1) / kSB;
+    // LG: physical rows of this lane's row li in each 16-row block of L11
+    int plr[LG ? NB : 1];
+#pragma unroll
+    for (int b = 0; b < (LG ? NB : 1); ++b) plr[b] = LG ? lds1<3>(rmap + j0 + min(kSB * b + li, jbl - 1)) : 0;
+    auto ldL = [&](const cx<T>* p) -> cx<T> { return LG ? ldc<1>(p) : ldc<3>(p); };
     // element (physical row prow, column col) of the trailing matrix: the
     // matrix part assembled from the operators at the first panel (ASM), the
     // right-hand-side columns (col >= n) from the slot; no branch around the loads
@@ -460,7 +470,9 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
                     for (int s = 0; s < 4; ++s) {
                         const int k = kSB * a + M::row(lg, s);
                         const int row = kSB * b + li;
-                        cx<T> l = ldc<3>(P + (min(row, jbl - 1) + (size_t)k * ldp));  // rows beyond: unstored
+                        // rows beyond: unstored
+                        cx<T> l = LG ? ldc<1>(S + (plr[b] + (size_t)(j0 + k) * ld))
+                                     : ldc<3>(P + (min(row, jbl - 1) + (size_t)k * ldp));
                         l.re = -l.re;
                         l.im = -l.im;
                         cmfma<T>(are[b], aim[b], l, ure[a][s], uim[a][s]);
@@ -468,7 +480,7 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
                 }
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
-                    const cx<T> li_v = ldc<3>(Linv + (kSB * kSB * b + li + M::row(lg, s) * kSB));
+                    const cx<T> li_v = ldL(Linv + (kSB * kSB * b + li + M::row(lg, s) * kSB));
                     cmfma<T>(ure[b], uim[b], li_v, are[b][s], aim[b][s]);
                 }
             }
@@ -531,7 +543,9 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
 #pragma unroll
                         for (int a = 0; a < TA; ++a) {
                             const int row = min(r0 + kSB * a + li, m2 - 1);
-                            lv[a] = ldc<3>(P + ((jbl + row) + (size_t)k * ldp));
+                            // LG: L21 shares its physical rows with the C tile
+                            lv[a] = LG ? ldc<1>(S + (prow[a] + (size_t)(j0 + k) * ld))
+                                       : ldc<3>(P + ((jbl + row) + (size_t)k * ldp));
                         }
                         if (G3) {
                             const T us = -(ure[b][s] + uim[b][s]);
@@ -584,7 +598,8 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
 }
 
 // ---- the wide-panel factorisation ------------------------------------------------------------
-template <typename T, int MAXR, bool G3>
+// TWO: two-level update (lay.jo columns per step); otherwise one level.
+template <typename T, int MAXR, bool G3, bool TWO>
 __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w2, cx<T>* __restrict__ S,
                                             const Layout& lay, const Lds& L, int nr) {
     const int n = s.n, tid = threadIdx.x;
@@ -612,50 +627,68 @@ __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w
     if (tid == 0) L.sc[1] = 0;
     __syncthreads();
 
-    for (int j0 = 0; j0 < n; j0 += jb) {
-        const int jbl = min(jb, n - j0), m = n - j0;
-        // complex64: odd leading dimension, so the columns an MFMA operand read
-        // spans start in different LDS banks (complex128 elements spread already)
-        const int ldp = sizeof(T) == 4 ? (m | 1) : m;
-        // ---- panel into LDS (logical rows), assembled from the operators at j0 = 0;
-        // every thread owns rows and keeps kPB columns of loads in flight
-        for (int r = tid; r < m; r += kThreads) {
-            const int pr = j0 == 0 ? r : lds1<3>(L.rmap + j0 + r);
-            for (int cb = 0; cb < jbl; cb += kPB) {
-                cx<T> v[kPB];
+    // two-level steps of jo > jb columns: the LDS panels of a step update only
+    // the step's own columns, and the columns right of it get one deferred
+    // rank-jo update with L read back from the slot after the step's last
+    // panel; the step holding the matrix's last columns updates everything
+    // panel by panel.  jo <= jb: one level.
+    const int jo = (TWO && lay.jo > jb) ? lay.jo : jb;
+    for (int J0 = 0; J0 < n; J0 += jo) {
+        const int jend = min(J0 + jo, n);
+        const bool defer = TWO && jo > jb && jend < n;
+        for (int j0 = J0; j0 < jend; j0 += jb) {
+            const int jbl = min(jb, n - j0), m = n - j0;
+            // complex64: odd leading dimension, so the columns an MFMA operand read
+            // spans start in different LDS banks (complex128 elements spread already)
+            const int ldp = sizeof(T) == 4 ? (m | 1) : m;
+            // ---- panel into LDS (logical rows), assembled from the operators at j0 = 0;
+            // every thread owns rows and keeps kPB columns of loads in flight
+            for (int r = tid; r < m; r += kThreads) {
+                const int pr = j0 == 0 ? r : lds1<3>(L.rmap + j0 + r);
+                for (int cb = 0; cb < jbl; cb += kPB) {
+                    cx<T> v[kPB];
 #pragma unroll
-                for (int u = 0; u < kPB; ++u) {
-                    const int c = min(cb + u, jbl - 1);
-                    v[u] = j0 == 0 ? to_t<T>(assemble_at(s, (size_t)pr + (size_t)c * n, w, w2))
-                                   : ldc<1>(S + (pr + (size_t)(j0 + c) * n));
+                    for (int u = 0; u < kPB; ++u) {
+                        const int c = min(cb + u, jbl - 1);
+                        v[u] = j0 == 0 ? to_t<T>(assemble_at(s, (size_t)pr + (size_t)c * n, w, w2))
+                                       : ldc<1>(S + (pr + (size_t)(j0 + c) * n));
+                    }
+#pragma unroll
+                    for (int u = 0; u < kPB; ++u)
+                        if (cb + u < jbl) stc<3>(P + (r + (size_t)(cb + u) * ldp), v[u]);
                 }
-#pragma unroll
-                for (int u = 0; u < kPB; ++u)
-                    if (cb + u < jbl) stc<3>(P + (r + (size_t)(cb + u) * ldp), v[u]);
+                // columns from jbl to the next multiple of 16 read as zero by the
+                // trailing update's last k-block
+                for (int c = jbl; c < ((jbl + kSB - 1) / kSB) * kSB; ++c) stc<3>(P + (r + (size_t)c * ldp), czero<T>());
             }
-            // columns from jbl to the next multiple of 16 read as zero by the
-            // trailing update's last k-block
-            for (int c = jbl; c < ((jbl + kSB - 1) / kSB) * kSB; ++c) stc<3>(P + (r + (size_t)c * ldp), czero<T>());
-        }
-        __syncthreads();
-        // ---- sub-panels
-        for (int c0 = 0; c0 < jbl; c0 += kSB) {
-            const int sbw = min(kSB, jbl - c0);
-            subpanel_factor<T, MAXR>(P, m, ldp, jbl, c0, sbw, j0, L.ipiv, L.rmap, L.sc, cand, mvl);
-            invert_unit_lower16<T>(P + c0 + (size_t)c0 * ldp, ldp, sbw, Linv + (size_t)(c0 / kSB) * kSB * kSB,
-                                   Lg + (size_t)((j0 + c0) / kSB) * kSB * kSB);
             __syncthreads();
-            if (c0 + kSB < jbl) inpanel_update<T>(P, m, ldp, jbl, c0, Linv + (size_t)(c0 / kSB) * kSB * kSB);
+            // ---- sub-panels
+            for (int c0 = 0; c0 < jbl; c0 += kSB) {
+                const int sbw = min(kSB, jbl - c0);
+                subpanel_factor<T, MAXR>(P, m, ldp, jbl, c0, sbw, j0, L.ipiv, L.rmap, L.sc, cand, mvl);
+                invert_unit_lower16<T>(P + c0 + (size_t)c0 * ldp, ldp, sbw, Linv + (size_t)(c0 / kSB) * kSB * kSB,
+                                       Lg + (size_t)((j0 + c0) / kSB) * kSB * kSB);
+                __syncthreads();
+                if (c0 + kSB < jbl) inpanel_update<T>(P, m, ldp, jbl, c0, Linv + (size_t)(c0 / kSB) * kSB * kSB);
+            }
+            // ---- the factored panel to the slot (rows at their physical positions)
+            for (int r = tid; r < m; r += kThreads) {
+                const int pr = lds1<3>(L.rmap + j0 + r);
+                for (int c = 0; c < jbl; ++c) stc<1>(S + (pr + (size_t)(j0 + c) * n), ldc<3>(P + (r + (size_t)c * ldp)));
+            }
+            // ---- trailing columns (only the step's own when deferring)
+            const int tcol = defer ? jend : n + nr;
+            if (j0 == 0) wide_trailing<T, true, G3, false>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
+            else wide_trailing<T, false, G3, false>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
+            __syncthreads();
         }
-        // ---- the factored panel to the slot (rows at their physical positions)
-        for (int r = tid; r < m; r += kThreads) {
-            const int pr = lds1<3>(L.rmap + j0 + r);
-            for (int c = 0; c < jbl; ++c) stc<1>(S + (pr + (size_t)(j0 + c) * n), ldc<3>(P + (r + (size_t)c * ldp)));
+        if (TWO && defer) {
+            // ---- deferred update of columns [jend, n + nr) by the step's jo columns
+            const cx<T>* Lgs = Lg + (size_t)(J0 / kSB) * kSB * kSB;
+            if (J0 == 0) wide_trailing<T, true, G3, true>(s, w, w2, S, n, L.rmap, J0, jo, n, n + nr, P, 0, Lgs);
+            else wide_trailing<T, false, G3, true>(s, w, w2, S, n, L.rmap, J0, jo, n, n + nr, P, 0, Lgs);
+            __syncthreads();
         }
-        // ---- trailing columns
-        if (j0 == 0) wide_trailing<T, true, G3>(s, w, w2, S, n, L.rmap, j0, jbl, n, n + nr, P, ldp, Linv);
-        else wide_trailing<T, false, G3>(s, w, w2, S, n, L.rmap, j0, jbl, n, n + nr, P, ldp, Linv);
-        __syncthreads();
     }
     if (sizeof(T) == 4) {
         uinv_blocks(S, n, L.rmap, kSB, Ug);

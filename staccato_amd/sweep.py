@@ -112,7 +112,8 @@ class Handle:
     def set_policy(self, n, mode, kernel=0, lds_budget=0, jb_max=0):
         """Launch policy for systems of order n (or the nearest order with an
         entry) in `mode`: kernel 0 automatic, 1 register panel, 2 wide LDS
-        panel, 3 wide LDS panel with Gauss 3M products (complex64); lds_budget
+        panel, 3 wide LDS panel with Gauss 3M products (complex64), 4 wide LDS
+        panel with the two-level update; lds_budget
         bytes of LDS per workgroup (0 automatic); jb_max
         widest panel (0 no cap).  None for kernel removes the entry."""
         if kernel is None:
