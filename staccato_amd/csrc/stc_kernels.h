@@ -1073,7 +1073,11 @@ __device__ __forceinline__ void lower_h_unit_solve(const TriCtx<T>& t, cx<T>* w,
 // rows outside the block from its prefetched 16 entries (further rows of
 // n > 256 load theirs on demand).  FWD: blocks in increasing order (lower
 // triangular op), else decreasing.  TRANS: op(LU)(i, k) = conj(LU(k, i)).
-template <typename T, bool FWD, bool INV, bool UNIT, bool TRANS>
+// BF: the inverse-apply and off-diagonal loops without branches (entries
+// beyond the block contribute zero, right-hand sides g >= gc repeat column
+// gc - 1 and are discarded, all 16 entries of a row are loaded); more
+// registers, used by the complex64 wide-panel kernel only.
+template <typename T, bool FWD, bool INV, bool UNIT, bool TRANS, bool BF = false>
 __device__ __forceinline__ void tri_solve_pipe(const TriCtx<T>& t, const cx<T>* __restrict__ binv, cx<T>* w,
                                                int gc) {
     constexpr int B = 16;
@@ -1105,8 +1109,8 @@ __device__ __forceinline__ void tri_solve_pipe(const TriCtx<T>& t, const cx<T>* 
         for (int u = 0; u < B; ++u) {
             // rows outside the update are loaded (clamped) but never used; entries
             // beyond kb (narrow blocks of the register-panel kernel) are not loaded
-            if (u < kb) {
-                const int k = k0 + u;
+            if (BF || u < kb) {
+                const int k = k0 + min(u, kb - 1);
                 r[u] = TRANS ? ldc<1>(LU + ((size_t)lds1<3>(t.rmap + k) + (size_t)i0c * ld))
                              : ldc<1>(LU + (pr0 + (size_t)k * ld));
             }
@@ -1135,8 +1139,16 @@ __device__ __forceinline__ void tri_solve_pipe(const TriCtx<T>& t, const cx<T>* 
                 cx<T>* y = w + (size_t)g * n + k0;
                 cx<T> acc = czero<T>();
 #pragma unroll
-                for (int k = 0; k < B; ++k)
-                    if (k < kb) cfma(acc, ldc<3>(Dg + (r + k * B)), ldc<3>(y + k));
+                for (int k = 0; k < B; ++k) {
+                    if (BF) {
+                        cx<T> d = ldc<3>(Dg + (r + k * B));
+                        d.re = k < kb ? d.re : T(0);
+                        d.im = k < kb ? d.im : T(0);
+                        cfma(acc, d, ldc<3>(y + min(k, kb - 1)));
+                    } else if (k < kb) {
+                        cfma(acc, ldc<3>(Dg + (r + k * B)), ldc<3>(y + k));
+                    }
+                }
                 if (r < kb) stc<3>(y + r, acc);  // one wave: every read above precedes the writes
             }
         } else {
@@ -1179,7 +1191,15 @@ __device__ __forceinline__ void tri_solve_pipe(const TriCtx<T>& t, const cx<T>* 
             for (int g = 0; g < kMaxGroup; ++g) acc[g] = czero<T>();
 #pragma unroll
             for (int u = 0; u < B; ++u) {
-                if (u < kb) {
+                if (BF) {
+                    cx<T> a = CONJ ? cconj(rc[u]) : rc[u];
+                    a.re = u < kb ? a.re : T(0);
+                    a.im = u < kb ? a.im : T(0);
+                    const int uc = min(u, kb - 1);
+#pragma unroll
+                    for (int g = 0; g < kMaxGroup; ++g)
+                        cfma(acc[g], a, ldc<3>(w + (k0 + uc + (size_t)min(g, gc - 1) * n)));
+                } else if (u < kb) {
                     const cx<T> a = CONJ ? cconj(rc[u]) : rc[u];
 #pragma unroll
                     for (int g = 0; g < kMaxGroup; ++g)
@@ -1260,7 +1280,11 @@ __device__ void block_sum(double (&vals)[NS], double* red) {
 // need_anorm), then per column g: out[1+4g] = ||r||_inf, out[2+4g] = ||x||_inf,
 // out[3+4g] = ||b||_inf, out[4+4g] = max |re r| + |im r|.
 // b is global (ldb) when bg != nullptr, else LDS (bl, ld n).
-template <bool ADJ, int RC>
+// BF: the real-operator rounds without branches (columns beyond n contribute
+// zero, right-hand sides g >= gc repeat column gc - 1 and are discarded), so
+// the compiler interleaves a round's loads with its arithmetic; it needs more
+// registers and is used by the complex64 wide-panel kernel only.
+template <bool ADJ, int RC, bool BF>
 __device__ __forceinline__ void residual(const SysDesc& s, double w, double w2, const double2* bg, int ldb,
                          const z64* bl, const z64* xs, z64* rs, int gc, bool need_anorm,
                          double* red, double (&out)[kRedSlots]) {
@@ -1291,14 +1315,31 @@ __device__ __forceinline__ void residual(const SysDesc& s, double w, double w2, 
                     mv[u] = lds1<1>(Mp + e);
                     dv[u] = s.has_D ? lds1<1>(Dp + e) : 0.0;
                 }
-#pragma unroll
-                for (int u = 0; u < RC; ++u) {
-                    if (j0 + u < n) {
-                        const z64 au = cmk<double>(kv[u] - w2 * mv[u], (ADJ ? -1.0 : 1.0) * (s.eta * kv[u] + w * dv[u]));
+                if (BF) {
+                    // branch-free: columns beyond n contribute zero, and columns
+                    // g >= gc repeat column gc - 1 (their sums are discarded)
+    #pragma unroll
+                    for (int u = 0; u < RC; ++u) {
+                        const bool in = j0 + u < n;
+                        const int j = min(j0 + u, n - 1);
+                        z64 au = cmk<double>(kv[u] - w2 * mv[u], (ADJ ? -1.0 : 1.0) * (s.eta * kv[u] + w * dv[u]));
+                        au.re = in ? au.re : 0.0;
+                        au.im = in ? au.im : 0.0;
                         if (need_anorm) asum += zabs(au);
-#pragma unroll
+    #pragma unroll
                         for (int g = 0; g < kMaxGroup; ++g)
-                            if (g < gc) cfms(acc[g], au, ldc<3>(xs + j0 + u + (size_t)g * n));
+                            cfms(acc[g], au, ldc<3>(xs + j + (size_t)min(g, gc - 1) * n));
+                    }
+                } else {
+    #pragma unroll
+                    for (int u = 0; u < RC; ++u) {
+                        if (j0 + u < n) {
+                            const z64 au = cmk<double>(kv[u] - w2 * mv[u], (ADJ ? -1.0 : 1.0) * (s.eta * kv[u] + w * dv[u]));
+                            if (need_anorm) asum += zabs(au);
+    #pragma unroll
+                            for (int g = 0; g < kMaxGroup; ++g)
+                                if (g < gc) cfms(acc[g], au, ldc<3>(xs + j0 + u + (size_t)g * n));
+                        }
                     }
                 }
                 continue;
@@ -1362,7 +1403,7 @@ __device__ __forceinline__ double backward_error(double rmax, double anorm, doub
 // On entry wv holds P b (permuted, type T) for gc columns and xs/rs are free.
 // On exit xs holds the solution (FP64).  Returns refinement steps; conv[] = 1
 // for columns meeting tol; berr_out[] their backward errors (when computed).
-template <typename T, bool ADJ>
+template <typename T, bool ADJ, bool BF = false>
 __device__ __forceinline__ int solve_group(const SysDesc& s, double w, double w2, const TriCtx<T>& tri,
                            const int* perm, cx<T>* wv, z64* xs, z64* rs, int gc,
                            const double2* bg, int ldb, const z64* bl, bool refine, bool want_berr,
@@ -1376,8 +1417,8 @@ __device__ __forceinline__ int solve_group(const SysDesc& s, double w, double w2
         if (tri.rmap) {  // factors in the global slot: pipelined block solves
             if (!ADJ) {
                 if (fwd_done) fwd_done = false;
-                else tri_solve_pipe<T, true, true, true, false>(tri, tri.Linv, wv, gc);
-                if (tri.inv_u) tri_solve_pipe<T, false, true, false, false>(tri, tri.Uinv, wv, gc);
+                else tri_solve_pipe<T, true, true, true, false, BF>(tri, tri.Linv, wv, gc);
+                if (tri.inv_u) tri_solve_pipe<T, false, true, false, false, BF>(tri, tri.Uinv, wv, gc);
                 else tri_solve_pipe<T, false, false, false, false>(tri, nullptr, wv, gc);
             } else {
                 tri_solve_pipe<T, true, false, false, true>(tri, nullptr, wv, gc);
@@ -1405,7 +1446,7 @@ __device__ __forceinline__ int solve_group(const SysDesc& s, double w, double w2
     if (!want_berr) return 0;
     for (;;) {
         double out[kRedSlots];
-        residual<ADJ, ResCols<T>::K>(s, w, w2, bg, ldb, bl, xs, rs, gc, *anorm < 0.0, red, out);
+        residual<ADJ, ResCols<T>::K, BF>(s, w, w2, bg, ldb, bl, xs, rs, gc, *anorm < 0.0, red, out);
         if (*anorm < 0.0) *anorm = out[0];
         bool conv = true;
 #pragma unroll
@@ -1737,7 +1778,9 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
         __syncthreads();
         double be[kMaxGroup];
         bool conv;
-        const int it = solve_group<T, false>(s, w, w2, tri, L.rmap, wv, xs, rs, gc, Fg, s.ldf,
+        // branch-free residual rounds in the complex64 wide-panel kernel (the
+        // others spill with them)
+        const int it = solve_group<T, false, WIDE != 0 && sizeof(T) == 4>(s, w, w2, tri, L.rmap, wv, xs, rs, gc, Fg, s.ldf,
                                              nullptr, refine, want_berr, kp.tol, kp.max_iter,
                                              &anorm, be, &conv, L.red, fwd);
         if (refine && !conv) ok = false;
