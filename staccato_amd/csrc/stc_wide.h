@@ -414,13 +414,16 @@ __device__ __forceinline__ void inpanel_update(cx<T>* __restrict__ P, int m, int
 // their inverted diagonal blocks in the slot: the deferred update of a
 // two-level step, whose jbl = 16..64 columns were factored as several LDS
 // panels (every 16-block full).
-template <typename T, bool ASM, bool G3, bool LG>
+// NBL > 0: the panel has exactly NBL 16-column blocks (jbl = 16 NBL), known at
+// compile time, so the block loops carry no branches and no unused registers.
+template <typename T, bool ASM, bool G3, bool LG, int NBL = 0>
 __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, double w2, cx<T>* __restrict__ S, int ld,
                                               const int* rmap, int j0, int jbl, int n, int ncol, const cx<T>* P,
                                               int ldp, const cx<T>* Linv) {
     using M = Mfma<T>;
     using acc_t = typename M::acc;
-    constexpr int NB = WideCfg<T>::NB;
+    constexpr int NB = NBL ? NBL : WideCfg<T>::NB;
+    if (NBL) jbl = kSB * NBL;
     constexpr int TA = WideCfg<T>::TA;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lg = lane >> 4;
@@ -597,6 +600,17 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
     }
 }
 
+// wide_trailing with the panel's block count fixed at compile time for the
+// common widths (16 and 32 columns).
+template <typename T, bool ASM, bool G3>
+__device__ __forceinline__ void wide_trailing_w(const SysDesc& sd, double w, double w2, cx<T>* __restrict__ S, int ld,
+                                                const int* rmap, int j0, int jbl, int n, int ncol, const cx<T>* P,
+                                                int ldp, const cx<T>* Linv) {
+    if (jbl == kSB) wide_trailing<T, ASM, G3, false, 1>(sd, w, w2, S, ld, rmap, j0, jbl, n, ncol, P, ldp, Linv);
+    else if (jbl == 2 * kSB) wide_trailing<T, ASM, G3, false, 2>(sd, w, w2, S, ld, rmap, j0, jbl, n, ncol, P, ldp, Linv);
+    else wide_trailing<T, ASM, G3, false>(sd, w, w2, S, ld, rmap, j0, jbl, n, ncol, P, ldp, Linv);
+}
+
 // ---- the wide-panel factorisation ------------------------------------------------------------
 // TWO: two-level update (lay.jo columns per step); otherwise one level.
 template <typename T, int MAXR, bool G3, bool TWO>
@@ -678,8 +692,8 @@ __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w
             }
             // ---- trailing columns (only the step's own when deferring)
             const int tcol = defer ? jend : n + nr;
-            if (j0 == 0) wide_trailing<T, true, G3, false>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
-            else wide_trailing<T, false, G3, false>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
+            if (j0 == 0) wide_trailing_w<T, true, G3>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
+            else wide_trailing_w<T, false, G3>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
             __syncthreads();
         }
         if (TWO && defer) {
