@@ -115,6 +115,26 @@ __device__ __forceinline__ void wave_argmax_packed(unsigned long long& hk, unsig
     hk = ((unsigned long long)h << 32) | l;
     lo = (unsigned)__builtin_amdgcn_readlane((int)lo, 63);
 }
+// complex64 keys fit one 64-bit word: the 32-bit magnitude key above the
+// 20-bit position word (lo < 2^20), so the lexicographic order is the integer
+// order and a DPP step moves two dwords instead of three.
+template <int CTRL, int RM>
+__device__ __forceinline__ void argmax_step64(unsigned long long& v) {
+    const unsigned long long o = ((unsigned long long)dppu<CTRL, RM>((unsigned)(v >> 32)) << 32) |
+                                 dppu<CTRL, RM>((unsigned)v);
+    v = o > v ? o : v;
+}
+__device__ __forceinline__ void wave_argmax64(unsigned long long& v) {
+    argmax_step64<kDppXor1, 0xF>(v);
+    argmax_step64<kDppXor2, 0xF>(v);
+    argmax_step64<kDppHalfMirror, 0xF>(v);
+    argmax_step64<kDppMirror, 0xF>(v);
+    argmax_step64<kDppRowBcast15, 0xA>(v);
+    argmax_step64<kDppRowBcast31, 0xC>(v);
+    const unsigned h = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), 63);
+    const unsigned l = (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+    v = ((unsigned long long)h << 32) | l;
+}
 __device__ __forceinline__ void stc1i(int* p, int v) { *(__attribute__((address_space(3))) int*)(p) = v; }
 __device__ __forceinline__ void stc1u(unsigned* p, unsigned v) { *(__attribute__((address_space(3))) unsigned*)(p) = v; }
 
@@ -170,7 +190,14 @@ __device__ __forceinline__ void subpanel_factor(cx<T>* __restrict__ P, int m, in
             hk = better ? k : hk;
             lo = better ? l : lo;
         }
-        wave_argmax_packed(hk, lo);
+        if (sizeof(T) == 4) {
+            unsigned long long v = (hk << 20) | lo;  // hk < 2^32 for complex64
+            wave_argmax64(v);
+            hk = v >> 20;
+            lo = (unsigned)(v & 0xFFFFFull);
+        } else {
+            wave_argmax_packed(hk, lo);
+        }
         const int wg = (int)(lo & 1023u);
         if (hk != 0ull && tid == (wg & (kThreads - 1))) {
             const int tw = wg / kThreads;
