@@ -190,11 +190,12 @@ __device__ __forceinline__ void subpanel_factor(cx<T>* __restrict__ P, int m, in
             hk = better ? k : hk;
             lo = better ? l : lo;
         }
+        unsigned long long pk = 0;  // complex64: key and position in one word
         if (sizeof(T) == 4) {
-            unsigned long long v = (hk << 20) | lo;  // hk < 2^32 for complex64
-            wave_argmax64(v);
-            hk = v >> 20;
-            lo = (unsigned)(v & 0xFFFFFull);
+            pk = (hk << 20) | lo;  // hk < 2^32 for complex64
+            wave_argmax64(pk);
+            hk = pk >> 20;
+            lo = (unsigned)(pk & 0xFFFFFull);
         } else {
             wave_argmax_packed(hk, lo);
         }
@@ -211,23 +212,39 @@ __device__ __forceinline__ void subpanel_factor(cx<T>* __restrict__ P, int m, in
             }
         }
         if (lane == 0) {
-            stc1u(cb + wave * 4, (unsigned)hk);
-            stc1u(cb + wave * 4 + 1, (unsigned)(hk >> 32));
-            stc1u(cb + wave * 4 + 2, lo);
+            if (sizeof(T) == 4) {
+                *(__attribute__((address_space(3))) unsigned long long*)(cb + wave * 4) = pk;
+            } else {
+                stc1u(cb + wave * 4, (unsigned)hk);
+                stc1u(cb + wave * 4 + 1, (unsigned)(hk >> 32));
+                stc1u(cb + wave * 4 + 2, lo);
+            }
         }
         __syncthreads();
         // ---- the pivot: best of the four waves' candidates
         unsigned long long bk = 0;
         unsigned bl = 0;
         int bw = 0;
+        if (sizeof(T) == 4) {
+            unsigned long long bp = 0;
 #pragma unroll
-        for (int v = 0; v < kWaves; ++v) {
-            const unsigned long long k = ((unsigned long long)lds1<3>(cb + v * 4 + 1) << 32) | lds1<3>(cb + v * 4);
-            const unsigned l = lds1<3>(cb + v * 4 + 2);
-            const bool better = k > bk || (k == bk && l > bl);
-            bk = better ? k : bk;
-            bl = better ? l : bl;
-            bw = better ? v : bw;
+            for (int v = 0; v < kWaves; ++v) {
+                const unsigned long long p = *(const __attribute__((address_space(3))) unsigned long long*)(cb + v * 4);
+                const bool better = p > bp;
+                bp = better ? p : bp;
+                bw = better ? v : bw;
+            }
+            bl = (unsigned)(bp & 0xFFFFFull);
+        } else {
+#pragma unroll
+            for (int v = 0; v < kWaves; ++v) {
+                const unsigned long long k = ((unsigned long long)lds1<3>(cb + v * 4 + 1) << 32) | lds1<3>(cb + v * 4);
+                const unsigned l = lds1<3>(cb + v * 4 + 2);
+                const bool better = k > bk || (k == bk && l > bl);
+                bk = better ? k : bk;
+                bl = better ? l : bl;
+                bw = better ? v : bw;
+            }
         }
         bw = __builtin_amdgcn_readfirstlane(bw);
         const int bpos = 1023 - (int)(__builtin_amdgcn_readfirstlane(bl) >> 10);
