@@ -234,6 +234,7 @@ bool layout_wide(SysDesc* sys, int nsys, int ts, int adj, int jb_cap, int maxr, 
         sys[i].lay_jb = L.jb;
         sys[i].lay_cw = L.cw;
         sys[i].lay_G = L.G;
+        sys[i].lay_rsg = L.rsg;
         // two-level update: steps of the widest panel the trailing update holds
         // (64 columns complex64, 32 complex128), a multiple of the LDS panel
         const int jmax = wide_jb_max(ts);
@@ -324,7 +325,9 @@ int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine
     size_t slot = 0;
     for (int i = 0; i < nsys; ++i)
         if (!sys[i].lay_small)
-            slot = std::max(slot, slot_elems_for(sys[i].n, cfg->wide ? kSB : sys[i].lay_jb));
+            slot = std::max(slot, slot_elems_for(sys[i].n, cfg->wide ? kSB : sys[i].lay_jb) +
+                                      (cfg->wide && sys[i].lay_rsg ? slot_rs_elems(sys[i].n, ts) : 0));
+    slot = (slot + 1) & ~(size_t)1;  // every slot 16-byte aligned (complex64 elements)
     cfg->slot_elems = slot;
     cfg->slot_bytes = slot * ts;
     if (slot && slot_budget > 0) {

@@ -71,7 +71,7 @@ struct SysDesc {
     int lay_cw;
     int lay_G;
     int lay_jo;          // wide path: outer width of the two-level update (<= lay_jb: none)
-    int pad2_;
+    int lay_rsg;         // wide path: FP64 residuals in the factor slot instead of LDS
 };
 
 struct KParams {
@@ -97,6 +97,7 @@ struct Layout {
     int G;          // right-hand sides per solve group
     int lds;        // bytes needed
     int jo = 0;     // wide path: columns per two-level step (<= jb: one level)
+    int rsg = 0;    // wide path: FP64 residuals in the factor slot (frees LDS for the group)
 };
 
 __host__ __device__ inline int align16(long long b) { return (int)((b + 15) & ~15LL); }
@@ -134,8 +135,14 @@ __host__ __device__ inline int lds_fixed_bytes(int n) {
 
 // Solve vectors: w (factor type), xs, rs (FP64) and, for the adjoint, the
 // kept forward solution and the adjoint right-hand side (FP64).
-__host__ __device__ inline int lds_solve_bytes(int n, int G, int ts, int adj) {
-    return align16((long long)n * G * ts) + (adj ? 4 : 2) * align16((long long)n * G * 16);
+__host__ __device__ inline int lds_solve_bytes(int n, int G, int ts, int adj, int rsg = 0) {
+    return align16((long long)n * G * ts) + ((adj ? 4 : 2) - (rsg ? 1 : 0)) * align16((long long)n * G * 16);
+}
+
+// Factor-slot elements (of ts bytes) kept for the FP64 residuals of one solve
+// group when they do not fit the LDS (wide path, lay_rsg).
+__host__ __device__ inline size_t slot_rs_elems(int n, int ts) {
+    return (size_t)n * kMaxGroup * 16 / ts + 1;  // + 1: 16-byte alignment for complex64
 }
 
 // Complex elements of the blocked path's LDS panel region: the n x jb panel,
@@ -225,10 +232,20 @@ inline Layout plan_layout_wide(int n, int nrhs, int ts, int budget, int jb_cap, 
         const long long need = fixed + wide_lds_bytes(n, jb, ts, maxr);
         if (need <= budget) {
             L.jb = jb; L.cw = kSB; L.lds = (int)need;
+            // the largest solve group, with the FP64 residuals in LDS if that
+            // group fits so, else in the factor slot
             for (int G = Gmax; G >= 1; --G) {
-                int ns = fixed + lds_solve_bytes(n, G, ts, adj) +
-                         align16((long long)kSolveBlk * kSolveBlk * ts);
-                if (ns <= budget) { L.G = G; if (ns > L.lds) L.lds = ns; return L; }
+                // the adjoint keeps them in LDS, and so do the kernels for n <= 256
+                // (maxr < 8), whose groups always fit
+                for (int rsg = 0; rsg <= ((adj || maxr < 8) ? 0 : 1); ++rsg) {
+                    const int ns = fixed + lds_solve_bytes(n, G, ts, adj, rsg) +
+                                   align16((long long)kSolveBlk * kSolveBlk * ts);
+                    if (ns <= budget) {
+                        L.G = G; L.rsg = rsg;
+                        if (ns > L.lds) L.lds = ns;
+                        return L;
+                    }
+                }
             }
             L.lds = 0x7fffffff;
             return L;

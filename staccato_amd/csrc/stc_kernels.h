@@ -1284,7 +1284,8 @@ __device__ void block_sum(double (&vals)[NS], double* red) {
 // zero, right-hand sides g >= gc repeat column gc - 1 and are discarded), so
 // the compiler interleaves a round's loads with its arithmetic; it needs more
 // registers and is used by the complex64 wide-panel kernel only.
-template <bool ADJ, int RC, bool BF>
+// RG: rs may be global memory (generic stores), else it is LDS.
+template <bool ADJ, int RC, bool BF, bool RG = false>
 __device__ __forceinline__ void residual(const SysDesc& s, double w, double w2, const double2* bg, int ldb,
                          const z64* bl, const z64* xs, z64* rs, int gc, bool need_anorm,
                          double* red, double (&out)[kRedSlots]) {
@@ -1367,7 +1368,10 @@ __device__ __forceinline__ void residual(const SysDesc& s, double w, double w2, 
         amax = nmax(amax, asum);
 #pragma unroll
         for (int g = 0; g < kMaxGroup; ++g)
-            if (g < gc) stc<3>(rs + i + (size_t)g * n, acc[g]);
+            if (g < gc) {
+                if (RG) rs[i + (size_t)g * n] = acc[g];
+                else stc<3>(rs + i + (size_t)g * n, acc[g]);
+            }
     }
     __syncthreads();
     // pass 2: the norms of the backward error
@@ -1403,7 +1407,7 @@ __device__ __forceinline__ double backward_error(double rmax, double anorm, doub
 // On entry wv holds P b (permuted, type T) for gc columns and xs/rs are free.
 // On exit xs holds the solution (FP64).  Returns refinement steps; conv[] = 1
 // for columns meeting tol; berr_out[] their backward errors (when computed).
-template <typename T, bool ADJ, bool BF = false>
+template <typename T, bool ADJ, bool BF = false, bool RG = false>
 __device__ __forceinline__ int solve_group(const SysDesc& s, double w, double w2, const TriCtx<T>& tri,
                            const int* perm, cx<T>* wv, z64* xs, z64* rs, int gc,
                            const double2* bg, int ldb, const z64* bl, bool refine, bool want_berr,
@@ -1446,7 +1450,7 @@ __device__ __forceinline__ int solve_group(const SysDesc& s, double w, double w2
     if (!want_berr) return 0;
     for (;;) {
         double out[kRedSlots];
-        residual<ADJ, ResCols<T>::K, BF>(s, w, w2, bg, ldb, bl, xs, rs, gc, *anorm < 0.0, red, out);
+        residual<ADJ, ResCols<T>::K, BF, RG>(s, w, w2, bg, ldb, bl, xs, rs, gc, *anorm < 0.0, red, out);
         if (*anorm < 0.0) *anorm = out[0];
         bool conv = true;
 #pragma unroll
@@ -1690,7 +1694,10 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
                                     (WIDE == 3 && lay.jo > lay.jb &&
                                      (lay.jo % lay.jb != 0 || lay.jo > kSB * WideCfg<T>::NB)))
                                  : (lay.jb > kWaves * panel_cols_per_wave((int)sizeof(cx<T>), MAXR) || lay.jb > 32);
-        if (kp.scratch == nullptr || kp.slot_elems < slot_elems_for(n, WIDE ? kSB : lay.jb) || bad_jb) {
+        if (kp.scratch == nullptr ||
+            kp.slot_elems < slot_elems_for(n, WIDE ? kSB : lay.jb) +
+                                (WIDE && s.lay_rsg ? slot_rs_elems(n, sizeof(cx<T>)) : 0) ||
+            bad_jb) {
             // no factor slot, or a panel wider than the register panel (the host
             // plans neither): report, write NaN, touch nothing else
             nan_outputs(s, fidx);
@@ -1740,10 +1747,21 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
     }
 
     const int G = lay.G;
-    const int nvec = (kp.flags & kAdjoint) ? 4 : 2;
+    // wide path: the FP64 residuals of a group go to the slot's tail when the
+    // host planned them there (lay_rsg: the group would not fit the LDS with them)
+    // (only the kernels for n > 256 carry the second copy of the solve loop)
+    constexpr bool kRsg = WIDE != 0 && MAXR >= 8;
+    const bool rsg = kRsg && !lay.small && s.lay_rsg;
+    const int nvec = ((kp.flags & kAdjoint) ? 4 : 2) - (rsg ? 1 : 0);
     cx<T>* wv = reinterpret_cast<cx<T>*>(solvep);
     z64* xs = reinterpret_cast<z64*>(solvep + align16((long long)n * G * sizeof(cx<T>)));
     z64* rs = reinterpret_cast<z64*>(reinterpret_cast<char*>(xs) + align16((long long)n * G * 16));
+    z64* rs_slot = nullptr;
+    if (rsg) {
+        const size_t nb16 = (size_t)((n + kSB - 1) / kSB);
+        const size_t off = ((size_t)n * (n + kMaxGroup) + 2 * nb16 * kSB * kSB + 1) & ~(size_t)1;
+        rs_slot = reinterpret_cast<z64*>(const_cast<cx<T>*>(LU) + off);
+    }
     char* after = reinterpret_cast<char*>(xs) + nvec * align16((long long)n * G * 16);
     TriCtx<T> tri;
     tri.LU = LU;
@@ -1779,10 +1797,18 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
         double be[kMaxGroup];
         bool conv;
         // branch-free residual rounds in the complex64 wide-panel kernel (the
-        // others spill with them)
-        const int it = solve_group<T, false, WIDE != 0 && sizeof(T) == 4>(s, w, w2, tri, L.rmap, wv, xs, rs, gc, Fg, s.ldf,
-                                             nullptr, refine, want_berr, kp.tol, kp.max_iter,
-                                             &anorm, be, &conv, L.red, fwd);
+        // others spill with them); residuals in LDS, or in the slot (a copy of
+        // the loop with generic accesses to them)
+        constexpr bool BF = WIDE != 0 && sizeof(T) == 4;
+        int it;
+        if (kRsg && rsg)
+            it = solve_group<T, false, BF, kRsg>(s, w, w2, tri, L.rmap, wv, xs, rs_slot, gc, Fg, s.ldf,
+                                                 nullptr, refine, want_berr, kp.tol, kp.max_iter,
+                                                 &anorm, be, &conv, L.red, fwd);
+        else
+            it = solve_group<T, false, BF, false>(s, w, w2, tri, L.rmap, wv, xs, rs, gc, Fg, s.ldf,
+                                                  nullptr, refine, want_berr, kp.tol, kp.max_iter,
+                                                  &anorm, be, &conv, L.red, fwd);
         if (refine && !conv) ok = false;
         if (it > it_max) it_max = it;
         if (X)
