@@ -468,7 +468,8 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
     using acc_t = typename M::acc;
     constexpr int NB = NBL ? NBL : WideCfg<T>::NB;
     if (NBL) jbl = kSB * NBL;
-    constexpr int TA = WideCfg<T>::TA;
+    // the assembling variant keeps half the C tile (its FP64 operator loads need the registers)
+    constexpr int TA = ASM ? (WideCfg<T>::TA > 2 ? 2 : WideCfg<T>::TA) : WideCfg<T>::TA;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lg = lane >> 4;
     const int c0 = j0 + jbl, m2 = n - c0;
