@@ -1800,6 +1800,7 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
         // others spill with them); residuals in LDS, or in the slot (a copy of
         // the loop with generic accesses to them)
         constexpr bool BF = WIDE != 0 && sizeof(T) == 4;
+        if (BF) __builtin_amdgcn_s_setprio(2);
         int it;
         if (kRsg && rsg)
             it = solve_group<T, false, BF, kRsg>(s, w, w2, tri, L.rmap, wv, xs, rs_slot, gc, Fg, s.ldf,
@@ -1809,6 +1810,7 @@ __device__ __forceinline__ void process_point(const SysDesc& s, int fidx, const 
             it = solve_group<T, false, BF, false>(s, w, w2, tri, L.rmap, wv, xs, rs, gc, Fg, s.ldf,
                                                   nullptr, refine, want_berr, kp.tol, kp.max_iter,
                                                   &anorm, be, &conv, L.red, fwd);
+        if (BF) __builtin_amdgcn_s_setprio(0);
         if (refine && !conv) ok = false;
         if (it > it_max) it_max = it;
         if (X)
