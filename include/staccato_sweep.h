@@ -113,7 +113,7 @@ int stc_device_ok(stc_handle h);
  * of four; complex64 only, complex128 launches run 2), 4 = as 2 with a two-level
  * update (the LDS panels of each 64-column step, 32 for complex128, update only
  * that step; the columns right of it get one deferred rank-64 / rank-32
- * update).  lds_budget: bytes of LDS per
+ * update; the automatic choice above n = 256).  lds_budget: bytes of LDS per
  * workgroup, 0 = automatic, else 16384 .. 163584 (the LDS the panel may use
  * sets the workgroups per CU and the panel width).  jb_max: widest panel
  * (0 = no cap).  An entry applies to launches whose largest system has order

@@ -93,7 +93,10 @@ def automatic_name(n, mode):
     complex64 = mode in ("refined", "fp32", 1)
     if n <= 128:
         return "register-panel"
-    return "wide-1perCU" if n > (512 if complex64 else 256) else "wide-2perCU"
+    one_per_cu = n > (512 if complex64 else 256)
+    if n > 256:  # the two-level update (complex128 beyond n = 512: register panel either way)
+        return "wide-2level-1perCU" if one_per_cu else "wide-2level-2perCU"
+    return "wide-1perCU" if one_per_cu else "wide-2perCU"
 
 
 def tune(handle, rom_id, freq_hz, F, mode, tol, sample=1024, reps=3, candidates=None,

@@ -460,7 +460,9 @@ __device__ __forceinline__ void inpanel_update(cx<T>* __restrict__ P, int m, int
 // panels (every 16-block full).
 // NBL > 0: the panel has exactly NBL 16-column blocks (jbl = 16 NBL), known at
 // compile time, so the block loops carry no branches and no unused registers.
-template <typename T, bool ASM, bool G3, bool LG, int NBL = 0>
+// NOC: U12 only (stored to the slot), no C update (the staged deferred update
+// applies C afterwards).
+template <typename T, bool ASM, bool G3, bool LG, int NBL = 0, bool NOC = false>
 __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, double w2, cx<T>* __restrict__ S, int ld,
                                               const int* rmap, int j0, int jbl, int n, int ncol, const cx<T>* P,
                                               int ldp, const cx<T>* Linv) {
@@ -542,7 +544,7 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
                 if (b < nbl && k < jbl && li < cw)
                     stc<1>(S + (lds1<3>(rmap + j0 + k) + (size_t)(cb + li) * ld), cmk<T>(ure[b][r], uim[b][r]));
             }
-        if (m2 <= 0) continue;
+        if (NOC || m2 <= 0) continue;
         // ---- C -= L21 U12, transposed: D = U12^T L21^T; register r of tile a holds
         // C[row 16 a + li][column row(lg, r)]
         __builtin_amdgcn_wave_barrier();
@@ -642,6 +644,154 @@ __device__ __forceinline__ void wide_trailing(const SysDesc& sd, double w, doubl
             }
         }
         __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---- deferred update of a two-level step, L21 staged through LDS -------------------------------
+// Columns [c0, ncol) with c0 = j0 + jo, rows [c0, n): C -= L21 U12 where the
+// step's columns [j0, j0 + jo) are factored in the slot.  U12 = L11^{-1} A12 is
+// computed per chunk first (wide_trailing without its C update); then, per
+// band of BR rows, the band's L21 (BR x jo) is staged in the LDS region Pb
+// (pcap complex elements, free between panels, odd leading dimension) and
+// every wavefront updates its chunks' C tiles of the band with U12 read back
+// from the slot.  Each C tile is read and written once per step instead of
+// once per LDS panel, and L21 is read once.  Needs pcap >= 17 jo.
+template <typename T, bool ASM, bool G3>
+__device__ __forceinline__ void wide_trailing_staged(const SysDesc& sd, double w, double w2, cx<T>* __restrict__ S,
+                                                     int ld, const int* rmap, int j0, int jo, int n, int ncol,
+                                                     cx<T>* Pb, long long pcap, const cx<T>* Linv) {
+    using M = Mfma<T>;
+    using acc_t = typename M::acc;
+    constexpr int NB = WideCfg<T>::NB;
+    constexpr int TA = ASM ? (WideCfg<T>::TA > 2 ? 2 : WideCfg<T>::TA) : WideCfg<T>::TA;
+    wide_trailing<T, ASM, G3, true, 0, true>(sd, w, w2, S, ld, rmap, j0, jo, n, ncol, Pb, 0, Linv);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int c0 = j0 + jo, m2 = n - c0;
+    const int nbl = jo / kSB;
+    const int nch = (ncol - c0 + kSB - 1) / kSB;
+    if (m2 <= 0) return;
+    // band height: a multiple of 64 rows when one fits, else of 16 (the tile
+    // loop clamps a partial band)
+    int BR = (int)(pcap / jo) / 16 * 16;
+    if (BR >= 64) BR = BR / 64 * 64;
+    while (BR > 16 && (long long)(BR | 1) * jo > pcap) BR -= (BR > 64 ? 64 : 16);
+    const int ldb = BR | 1;
+    auto load_el = [&](int prow, int col) -> cx<T> {
+        if (!ASM) return ldc<1>(S + (prow + (size_t)col * ld));
+        const cx<T> a = to_t<T>(assemble_at(sd, (size_t)prow + (size_t)min(col, n - 1) * ld, w, w2));
+        const cx<T> f = ldc<1>(S + (prow + (size_t)max(col, n) * ld));
+        return col < n ? a : f;
+    };
+    for (int rb = 0; rb < m2; rb += BR) {
+        const int nr = min(BR, m2 - rb);
+        __syncthreads();  // U12 stored, the previous band consumed
+        for (int e = tid; e < nr * jo; e += kThreads) {
+            const int r = e % nr, k = e / nr;
+            stc<3>(Pb + (r + (size_t)k * ldb), ldc<1>(S + (lds1<3>(rmap + c0 + rb + r) + (size_t)(j0 + k) * ld)));
+        }
+        __syncthreads();
+        for (int ch = wave; ch < nch; ch += kWaves) {
+            const int cb = c0 + ch * kSB, cw = min(kSB, ncol - cb);
+            // U12 of the chunk back from the slot, operand layout (lane: U12[k][li])
+            acc_t ure[NB], uim[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int k = kSB * b + M::row(lg, q);
+                    const cx<T> v = ldc<1>(S + (lds1<3>(rmap + j0 + min(k, jo - 1)) + (size_t)(cb + min(li, cw - 1)) * ld));
+                    const bool ok = b < nbl;
+                    ure[b][q] = ok ? v.re : T(0);
+                    uim[b][q] = ok ? v.im : T(0);
+                }
+            auto load_c = [&](int r0, acc_t (&cre)[TA], acc_t (&cim)[TA], int (&pr)[TA]) {
+#pragma unroll
+                for (int a = 0; a < TA; ++a) {
+                    const int row = rb + min(r0 + kSB * a + li, nr - 1);
+                    pr[a] = lds1<3>(rmap + c0 + row);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const cx<T> v = load_el(pr[a], cb + min(M::row(lg, r), cw - 1));
+                        cre[a][r] = v.re;
+                        cim[a][r] = v.im;
+                    }
+                }
+            };
+            acc_t cre[TA], cim[TA];
+            acc_t cy[G3 ? TA : 1];
+            int prow[TA];
+            load_c(0, cre, cim, prow);
+            for (int r0 = 0; r0 < nr; r0 += kSB * TA) {
+                acc_t nre[TA], nim[TA];
+                int nprow[TA];
+                const bool more = r0 + kSB * TA < nr;
+                if (more && !ASM) load_c(r0 + kSB * TA, nre, nim, nprow);
+                if (G3) {
+#pragma unroll
+                    for (int a = 0; a < TA; ++a) {
+                        acc_zero<T>(cy[a]);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) cim[a][r] += cre[a][r];
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    if (b < nbl) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int k = kSB * b + M::row(lg, q);
+                            const cx<T> ua = cmk<T>(-ure[b][q], -uim[b][q]);
+                            cx<T> lv[TA];
+#pragma unroll
+                            for (int a = 0; a < TA; ++a)
+                                lv[a] = ldc<3>(Pb + (min(r0 + kSB * a + li, nr - 1) + (size_t)k * ldb));
+                            if (G3) {
+                                const T us = -(ure[b][q] + uim[b][q]);
+#pragma unroll
+                                for (int a = 0; a < TA; ++a)
+                                    cmfma3<T>(cre[a], cy[a], cim[a], ua, us, lv[a].re, lv[a].im);
+                            } else {
+#pragma unroll
+                                for (int a = 0; a < TA; ++a) cmfma<T>(cre[a], cim[a], ua, lv[a].re, lv[a].im);
+                            }
+                        }
+                    }
+                }
+                if (G3) {
+#pragma unroll
+                    for (int a = 0; a < TA; ++a)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const T x = cre[a][r], y = cy[a][r];
+                            cre[a][r] = x - y;
+                            cim[a][r] = cim[a][r] - x - y;
+                        }
+                }
+#pragma unroll
+                for (int a = 0; a < TA; ++a) {
+                    if (r0 + kSB * a + li < nr) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int col = M::row(lg, r);
+                            if (col < cw) stc<1>(S + (prow[a] + (size_t)(cb + col) * ld), cmk<T>(cre[a][r], cim[a][r]));
+                        }
+                    }
+                }
+                if (more) {
+                    if (ASM) {
+                        load_c(r0 + kSB * TA, cre, cim, prow);
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < TA; ++a) {
+                            cre[a] = nre[a];
+                            cim[a] = nim[a];
+                            prow[a] = nprow[a];
+                        }
+                    }
+                }
+            }
+        }
     }
 }
 
@@ -749,8 +899,11 @@ __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w
         if (TWO && defer) {
             // ---- deferred update of columns [jend, n + nr) by the step's jo columns
             const cx<T>* Lgs = Lg + (size_t)(J0 / kSB) * kSB * kSB;
-            if (J0 == 0) wide_trailing<T, true, G3, true>(s, w, w2, S, n, L.rmap, J0, jo, n, n + nr, P, 0, Lgs);
-            else wide_trailing<T, false, G3, true>(s, w, w2, S, n, L.rmap, J0, jo, n, n + nr, P, 0, Lgs);
+            // L21 staged through the panel's LDS region ((n | 1) jb >= 17 jo always:
+            // jb >= 16, jo <= 4 jb, n >= jo)
+            const long long pcap = (long long)(n | 1) * jb;
+            if (J0 == 0) wide_trailing_staged<T, true, G3>(s, w, w2, S, n, L.rmap, J0, jo, n, n + nr, P, pcap, Lgs);
+            else wide_trailing_staged<T, false, G3>(s, w, w2, S, n, L.rmap, J0, jo, n, n + nr, P, pcap, Lgs);
             __syncthreads();
         }
     }
