@@ -892,8 +892,15 @@ __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w
             }
             // ---- trailing columns (only the step's own when deferring)
             const int tcol = defer ? jend : n + nr;
-            if (j0 == 0) wide_trailing_w<T, true, G3>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
-            else wide_trailing_w<T, false, G3>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
+            // (the two-level kernel keeps one variant of its narrow within-step update:
+            // less code, fewer spills)
+            if (TWO) {
+                if (j0 == 0) wide_trailing<T, true, G3, false>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
+                else wide_trailing<T, false, G3, false>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
+            } else {
+                if (j0 == 0) wide_trailing_w<T, true, G3>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
+                else wide_trailing_w<T, false, G3>(s, w, w2, S, n, L.rmap, j0, jbl, n, tcol, P, ldp, Linv);
+            }
             __syncthreads();
         }
         if (TWO && defer) {
