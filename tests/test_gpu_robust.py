@@ -9,7 +9,9 @@
 - a matrix full of NaN pivots like LAPACK izamax (first row) without
   disturbing the other matrices of the batch;
 - launches of one handle on two streams are ordered;
-- the tensor checks of the device wrappers.
+- the tensor checks of the device wrappers;
+- the two-level update (the default above n = 256) with partial steps and
+  bands, many right-hand sides and the adjoint.
 Checked against the closed-form modal solution (staccato_amd/synth.py) and
 numpy's LAPACK solve.
 """
@@ -214,3 +216,51 @@ def test_device_wrappers_check_tensors(handle):
                  (A.transpose(1, 2), B, Xb, ib)]:
         with pytest.raises((ValueError, TypeError)):
             handle.gesv_batched_dev(*args)
+
+
+# ---- the two-level update (the default above n = 256) on its edge cases ----------------
+
+@pytest.mark.parametrize("mode", ["plain", "refined"])
+def test_two_level_partial_steps_many_rhs_and_mixed_batch(handle, mode):
+    """n = 300 (last step and last band partial) and n = 512 with 16
+    right-hand sides (complex64: two workgroups per CU, groups of four with
+    the FP64 residuals in the factor slot), next to small systems in the same
+    launch."""
+    rng = np.random.default_rng(17)
+    systems, cases = [], []
+    for i, (n, nrhs) in enumerate(((512, 16), (300, 3), (64, 2), (130, 5))):
+        rom, r2 = synth.modal_rom(n, 900 + n, 0.02, alpha=1e-4, beta=0.3)
+        c = synth.Case(rom, np.linspace(2.0, 1600.0, 11), synth.complex_normal(r2, (n, nrhs)))
+        handle.rom_set(470 + i, rom.K, rom.M, rom.D, rom.eta)
+        systems.append((470 + i, c.freq, c.F))
+        cases.append(c)
+    tol = 1e-10 if mode == "refined" else 1e-12
+    outs, st = handle.sweep_batch(systems, mode=mode, tol=tol)
+    for (X, info, be), c in zip(outs, cases):
+        assert (info == 0).all(), (c.rom.n, info)
+        assert be.max() <= tol, (c.rom.n, be.max())
+        ok, worst = _closed_ok(X, c, 4 * tol if mode == "refined" else 1e-13)
+        assert ok, (c.rom.n, worst)
+    # one system alone (its own launch) agrees with the batch to the tolerance
+    X1, info1, be1, _ = handle.sweep(471, cases[1].freq, cases[1].F, mode=mode, tol=tol)
+    assert (info1 == 0).all()
+    Xb = outs[1][0]
+    assert np.abs(X1 - Xb).max() <= 1e3 * tol * np.abs(Xb).max()
+
+
+@pytest.mark.parametrize("mode,tol", [("plain", 0.0), ("refined", 1e-12)])
+def test_adjoint_on_the_two_level_path(handle, mode, tol):
+    """The fused adjoint of an n = 300 ROM (two-level factorisation, residuals
+    in LDS) against the closed-form modal gradient."""
+    rom, rng = synth.modal_rom(300, 31, 0.02)
+    freq = np.linspace(1.0, 900.0, 64)
+    F = synth.complex_normal(rng, (300, 1))
+    cv = synth.complex_normal(rng, (300,))
+    handle.rom_set(480, rom.K, rom.M, rom.D, rom.eta)
+    J, ds, de, info = handle.adjoint(480, freq, F, cv, rom.Phi, rom.kdiag, mode=mode, tol=tol)
+    assert (info == 0).all()
+    Jc, dsc, dec = synth.adjoint_closed_form(rom, freq, F, cv)
+    gate = 1e-8 if mode == "plain" else 1e-6
+    assert abs(J / Jc - 1) < gate
+    assert np.abs(ds - dsc).max() <= gate * np.abs(dsc).max()
+    assert abs(de / dec - 1) < gate
