@@ -30,6 +30,7 @@
 #include "stc_internal.h"
 
 #include <type_traits>
+#include <utility>
 
 namespace stc {
 

@@ -138,6 +138,55 @@ __device__ __forceinline__ void wave_argmax64(unsigned long long& v) {
 __device__ __forceinline__ void stc1i(int* p, int v) { *(__attribute__((address_space(3))) int*)(p) = v; }
 __device__ __forceinline__ void stc1u(unsigned* p, unsigned v) { *(__attribute__((address_space(3))) unsigned*)(p) = v; }
 
+// Rows moved by a sub-panel's pivoting (the list mvl, sc[12] entries): their
+// entries in the other panel columns and in the row map follow; every source
+// is read before any destination is written.
+template <typename T>
+__device__ __forceinline__ void subpanel_moves(cx<T>* __restrict__ P, int ldp, int jbl, int c0, int sbw, int j0,
+                                               int* rmap, int* sc, int2* mvl) {
+    const int tid = threadIdx.x;
+    __syncthreads();
+    // permute the moved rows of the other panel columns and of the row map:
+    // read every source, then write every destination
+    const int cnt = uniform(lds1<3>(sc + 12));
+    const int nother = jbl - sbw;  // panel columns outside the sub-panel
+    const int per = nother + 1;    // + the row-map entry
+    const int total = cnt * per;
+    constexpr int kMv = 8;         // cnt <= 2 sbw = 32, per <= 49: <= 1568 = 6.1 per lane
+    cx<T> tv[kMv];
+    int ti[kMv];
+#pragma unroll
+    for (int u = 0; u < kMv; ++u) {
+        const int e = tid + kThreads * u;
+        if (e < total) {
+            const int i = e / per, k = e - i * per;
+            const int2 mvp = mvl[i];
+            if (k == nother) {
+                ti[u] = rmap[j0 + mvp.x];
+            } else {
+                const int c = k < c0 ? k : k + sbw;
+                tv[u] = ldc<3>(P + (mvp.x + (size_t)c * ldp));
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kMv; ++u) {
+        const int e = tid + kThreads * u;
+        if (e < total) {
+            const int i = e / per, k = e - i * per;
+            const int2 mvp = mvl[i];
+            if (k == nother) {
+                rmap[j0 + mvp.y] = ti[u];
+            } else {
+                const int c = k < c0 ? k : k + sbw;
+                stc<3>(P + (mvp.y + (size_t)c * ldp), tv[u]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // ---- 16-column sub-panel of the LDS panel ------------------------------------------------
 // Columns [c0, c0 + sbw) of P (m x jb, ld m, logical row order), rows
 // [c0, m).  Every thread holds whole rows of the sub-panel (rows
@@ -310,46 +359,116 @@ __device__ __forceinline__ void subpanel_factor(cx<T>* __restrict__ P, int m, in
             }
         }
     }
-    __syncthreads();
-    // permute the moved rows of the other panel columns and of the row map:
-    // read every source, then write every destination
-    const int cnt = uniform(lds1<3>(sc + 12));
-    const int nother = jbl - sbw;  // panel columns outside the sub-panel
-    const int per = nother + 1;    // + the row-map entry
-    const int total = cnt * per;
-    constexpr int kMv = 8;         // cnt <= 2 sbw = 32, per <= 49: <= 1568 = 6.1 per lane
-    cx<T> tv[kMv];
-    int ti[kMv];
+    subpanel_moves<T>(P, ldp, jbl, c0, sbw, j0, rmap, sc, mvl);
+}
+
+// f(integral_constant<int, J>) for each J of the sequence, in order
+template <typename F, int... J>
+__device__ __forceinline__ void static_for(F&& f, std::integer_sequence<int, J...>) {
+    (f(std::integral_constant<int, J>{}), ...);
+}
+
+// ---- one-wavefront sub-panel (complex64, at most 64 RW rows) --------------------------------
+// The same factorisation as subpanel_factor, with every row of the sub-panel
+// in wave 0's registers: lane l holds rows l + 64 t (t < RW), x[c][t] =
+// column c.  The column loop is unrolled, so no rotation is needed; per column
+// the argmax is one DPP reduction (the same packed keys and tie-breaking), the
+// winner's lane and register come back uniform, and the pivot row reaches
+// every lane by readlane -- no LDS exchange and no barrier per column.  The
+// other waves wait at subpanel_moves' first barrier.
+template <int RW>
+__device__ __forceinline__ void subpanel_factor_1w(cx<float>* __restrict__ P, int m, int ldp, int jbl, int c0, int sbw,
+                                                   int j0, int* ipiv, int* rmap, int* sc, int2* mvl) {
+    static_assert(RW >= 1 && RW <= 4, "one wavefront holds at most 256 rows");
+    const int tid = threadIdx.x;
+    const int mq = m - c0;
+    cx<float>* Q = P + c0 + (size_t)c0 * ldp;  // sub-panel origin, ld ldp
+    if (tid < 64) {
+        const int lane = tid;
+        cx<float> x[kSB][RW];
+        int lpos[RW];  // logical position in the sub-panel; -1: no row
 #pragma unroll
-    for (int u = 0; u < kMv; ++u) {
-        const int e = tid + kThreads * u;
-        if (e < total) {
-            const int i = e / per, k = e - i * per;
-            const int2 mvp = mvl[i];
-            if (k == nother) {
-                ti[u] = rmap[j0 + mvp.x];
-            } else {
-                const int c = k < c0 ? k : k + sbw;
-                tv[u] = ldc<3>(P + (mvp.x + (size_t)c * ldp));
+        for (int t = 0; t < RW; ++t) {
+            const int r = lane + 64 * t;
+            lpos[t] = r < mq ? r : -1;
+#pragma unroll
+            for (int c = 0; c < kSB; ++c)
+                x[c][t] = ldz<3>(Q + (min(r, mq - 1) + (size_t)min(c, sbw - 1) * ldp), c < sbw && r < mq);
+        }
+        if (lane == 0) stc1i(sc + 12, 0);
+        // column jj of the sub-panel; instantiated for jj = 0 .. 15 so that every
+        // register index is static
+        auto column = [&](auto jc) {
+            constexpr int jj = decltype(jc)::value;
+            if (jj >= sbw) return;
+            unsigned long long pk = 0;
+#pragma unroll
+            for (int t = 0; t < RW; ++t) {
+                const bool elig = lpos[t] >= jj;
+                const unsigned long long k =
+                    elig ? (pivot_ukey(cabs1(x[jj][t]), lpos[t] == jj) << 20) |
+                               ((unsigned)(1023 - lpos[t]) << 10) | (unsigned)(lane + 64 * t)
+                         : 0ull;
+                pk = k > pk ? k : pk;
+            }
+            wave_argmax64(pk);
+            const unsigned lo = (unsigned)(pk & 0xFFFFFull);
+            const int lw = (int)(lo & 63u), tw = (int)((lo >> 6) & 15u);
+            const int bpos = 1023 - (int)(lo >> 10);
+            // the pivot row, columns jj .. 15, uniform in every lane
+            cx<float> uv[kSB];
+#pragma unroll
+            for (int c = 0; c < kSB; ++c) uv[c] = czero<float>();
+#pragma unroll
+            for (int t = 0; t < RW; ++t) {
+                if (tw == t) {
+#pragma unroll
+                    for (int c = jj; c < kSB; ++c) {
+                        uv[c].re = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x[c][t].re), lw));
+                        uv[c].im = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x[c][t].im), lw));
+                    }
+                }
+            }
+            const cx<float> pv = uv[jj];
+            const bool zero = pv.re == 0.f && pv.im == 0.f;
+            const cx<float> rp = zero ? czero<float>() : crecip(pv);
+            if (lane == 0) {
+                stc1i(ipiv + j0 + c0 + jj, j0 + c0 + bpos);
+                if (zero && lds1<3>(sc + 1) == 0) stc1i(sc + 1, j0 + c0 + jj + 1);
+            }
+#pragma unroll
+            for (int t = 0; t < RW; ++t) {
+                if (lpos[t] == jj) lpos[t] = bpos;
+                else if (lpos[t] == bpos) lpos[t] = jj;
+            }
+            // multipliers and rank-1 update of the rows below the pivot
+#pragma unroll
+            for (int t = 0; t < RW; ++t) {
+                if (lpos[t] > jj) {
+                    const cx<float> l = zero ? x[jj][t] : cmul(x[jj][t], rp);
+                    x[jj][t] = l;
+#pragma unroll
+                    for (int c = jj + 1; c < kSB; ++c) cfms(x[c][t], l, uv[c]);
+                }
+            }
+        };
+        static_for(column, std::make_integer_sequence<int, kSB>{});
+        // the sub-panel at its new rows and the list of moved rows
+#pragma unroll
+        for (int t = 0; t < RW; ++t) {
+            const int r = lane + 64 * t;
+            if (r < mq) {
+#pragma unroll
+                for (int c = 0; c < kSB; ++c)
+                    if (c < sbw) stc<3>(Q + (lpos[t] + (size_t)c * ldp), x[c][t]);
+                if (lpos[t] != r) {
+                    const int idx = atomicAdd(sc + 12, 1);
+                    mvl[idx] = make_int2(c0 + r, c0 + lpos[t]);
+                }
             }
         }
     }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < kMv; ++u) {
-        const int e = tid + kThreads * u;
-        if (e < total) {
-            const int i = e / per, k = e - i * per;
-            const int2 mvp = mvl[i];
-            if (k == nother) {
-                rmap[j0 + mvp.y] = ti[u];
-            } else {
-                const int c = k < c0 ? k : k + sbw;
-                stc<3>(P + (mvp.y + (size_t)c * ldp), tv[u]);
-            }
-        }
-    }
-    __syncthreads();
+    subpanel_moves<float>(P, ldp, jbl, c0, sbw, j0, rmap, sc, mvl);
 }
 
 // ---- inverse of a unit-lower 16 x 16 diagonal block ----------------------------------------
@@ -878,7 +997,16 @@ __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w
                 // take issue priority over the other workgroup's update on the CU
                 // (complex64: +0.6 %; complex128 measured -0.6 % and keeps 0)
                 if (sizeof(T) == 4) __builtin_amdgcn_s_setprio(2);
-                subpanel_factor<T, MAXR>(P, m, ldp, jbl, c0, sbw, j0, L.ipiv, L.rmap, L.sc, cand, mvl);
+                // complex64 sub-panels of at most 64 rows: one wavefront, no
+                // barrier per column (measured at n = 256: +2.3 %; with up to 256
+                // rows in one wavefront its serial rank-1 updates cost about what
+                // the barriers save)
+                bool one_wave = false;
+                if constexpr (sizeof(T) == 4) {
+                    one_wave = m - c0 <= 64;
+                    if (one_wave) subpanel_factor_1w<1>(P, m, ldp, jbl, c0, sbw, j0, L.ipiv, L.rmap, L.sc, mvl);
+                }
+                if (!one_wave) subpanel_factor<T, MAXR>(P, m, ldp, jbl, c0, sbw, j0, L.ipiv, L.rmap, L.sc, cand, mvl);
                 if (sizeof(T) == 4) __builtin_amdgcn_s_setprio(0);
                 invert_unit_lower16<T>(P + c0 + (size_t)c0 * ldp, ldp, sbw, Linv + (size_t)(c0 / kSB) * kSB * kSB,
                                        Lg + (size_t)((j0 + c0) / kSB) * kSB * kSB);
