@@ -110,7 +110,8 @@ int stc_device_ok(stc_handle h);
  * complex64, two for complex128), 2 = wide panel held in LDS (16..64 columns,
  * one or two workgroups per CU, by lds_budget), 3 = as 2 with the trailing
  * update's complex products in Gauss 3M form (three real MFMA products instead
- * of four; complex64 only, complex128 launches run 2), 4 = as 2 with a two-level
+ * of four; complex64 only, complex128 launches run 2; the automatic choice
+ * for complex64 from n = 129 to 256), 4 = as 2 with a two-level
  * update (the LDS panels of each 64-column step, 32 for complex128, update only
  * that step; the columns right of it get one deferred rank-64 / rank-32
  * update; the automatic choice above n = 256).  lds_budget: bytes of LDS per

@@ -96,6 +96,8 @@ def automatic_name(n, mode):
     one_per_cu = n > (512 if complex64 else 256)
     if n > 256:  # the two-level update (complex128 beyond n = 512: register panel either way)
         return "wide-2level-1perCU" if one_per_cu else "wide-2level-2perCU"
+    if complex64:  # Gauss 3M products up to n = 256
+        return "wide-3M-2perCU"
     return "wide-1perCU" if one_per_cu else "wide-2perCU"
 
 

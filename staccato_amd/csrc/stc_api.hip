@@ -299,9 +299,12 @@ int plan_launch(stc_ctx* h, SysDesc* sys, int nsys, long long nitems, int refine
     cfg->wide = 0;
     // automatic: the two-level update above n = 256 (measured on MI355X, both
     // modes: n = 384 +11 % / +7 %, 512 +24 % / +12 %, 1024 +36 % complex64),
-    // one level up to n = 256 (there two-level loses: 310k against 438k points/s)
+    // one level up to n = 256 (there two-level loses for complex64: 374k
+    // against 457k points/s at n = 256), with Gauss 3M products for complex64
+    // (n = 160 / 192 / 224 / 256: +1.5 / +5.3 / +4.8 / +3.5 % over 4M)
     const bool two_auto = pol.kernel == 0 && maxn > 256;
-    const int wkind = (pol.kernel == 3 && sweep_kernel_fn(refine, maxr, 2)) ? 2
+    const bool g3_auto = pol.kernel == 0 && refine && maxn <= 256;
+    const int wkind = ((pol.kernel == 3 || g3_auto) && sweep_kernel_fn(refine, maxr, 2)) ? 2
                       : ((pol.kernel == 4 || two_auto) && sweep_kernel_fn(refine, maxr, 3)) ? 3 : 1;
     if (((pol.kernel == 0 && blocked && maxn > 128) || pol.kernel >= 2) && sweep_kernel_fn(refine, maxr, wkind)) {
         int wcap = wide_jb_max(ts);
