@@ -1076,25 +1076,27 @@ __device__ __forceinline__ void factor_wide(const SysDesc& s, double w, double w
             // complex64: odd leading dimension, so the columns an MFMA operand read
             // spans start in different LDS banks (complex128 elements spread already)
             const int ldp = sizeof(T) == 4 ? (m | 1) : m;
-            // ---- panel into LDS (logical rows), assembled from the operators at j0 = 0;
-            // every thread owns rows and keeps kPB columns of loads in flight
-            for (int r = tid; r < m; r += kThreads) {
+            // ---- panel into LDS (logical rows), assembled from the operators at j0 = 0:
+            // items of kPB columns of one row (loads in flight together), spread over
+            // all threads, so short panels keep every thread loading
+            const int nbat = (jbl + kPB - 1) / kPB;
+            for (int e = tid; e < m * nbat; e += kThreads) {
+                const int r = e % m, cb = (e / m) * kPB;
                 const int pr = j0 == 0 ? r : lds1<3>(L.rmap + j0 + r);
-                for (int cb = 0; cb < jbl; cb += kPB) {
-                    cx<T> v[kPB];
+                cx<T> v[kPB];
 #pragma unroll
-                    for (int u = 0; u < kPB; ++u) {
-                        const int c = min(cb + u, jbl - 1);
-                        v[u] = j0 == 0 ? to_t<T>(assemble_at(s, (size_t)pr + (size_t)c * n, w, w2))
-                                       : ldc<1>(S + (pr + (size_t)(j0 + c) * n));
-                    }
-#pragma unroll
-                    for (int u = 0; u < kPB; ++u)
-                        if (cb + u < jbl) stc<3>(P + (r + (size_t)(cb + u) * ldp), v[u]);
+                for (int u = 0; u < kPB; ++u) {
+                    const int c = min(cb + u, jbl - 1);
+                    v[u] = j0 == 0 ? to_t<T>(assemble_at(s, (size_t)pr + (size_t)c * n, w, w2))
+                                   : ldc<1>(S + (pr + (size_t)(j0 + c) * n));
                 }
+#pragma unroll
+                for (int u = 0; u < kPB; ++u)
+                    if (cb + u < jbl) stc<3>(P + (r + (size_t)(cb + u) * ldp), v[u]);
                 // columns from jbl to the next multiple of 16 read as zero by the
                 // trailing update's last k-block
-                for (int c = jbl; c < ((jbl + kSB - 1) / kSB) * kSB; ++c) stc<3>(P + (r + (size_t)c * ldp), czero<T>());
+                if (cb == 0)
+                    for (int c = jbl; c < ((jbl + kSB - 1) / kSB) * kSB; ++c) stc<3>(P + (r + (size_t)c * ldp), czero<T>());
             }
             __syncthreads();
             // ---- sub-panels
