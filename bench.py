@@ -40,11 +40,11 @@ PEAK_HBM_GBS = 8000.0
 # commands are in profiles/sweep_kernel_n256_pmc.txt; re-measure when the kernel changes.
 TRAFFIC_KIB = {
     ("refined", "wide-2perCU"): (7.035e6, 8.095e6),
-    ("refined", "wide-3M-2perCU"): (6.987e6, 7.977e6),
+    ("refined", "wide-3M-2perCU"): (7.025e6, 7.991e6),
     ("refined", "wide-1perCU-jb32"): (6.900e6, 8.069e6),
     ("refined", "register-panel"): (11.23e6, 14.23e6),
     ("plain", "wide-2perCU"): (15.12e6, 26.92e6),
-    ("plain", "wide-2level-2perCU"): (14.04e6, 16.04e6),
+    ("plain", "wide-2level-2perCU"): (13.98e6, 16.01e6),
     ("plain", "wide-1perCU"): (7.217e6, 14.84e6),
     ("plain", "register-panel"): (19.79e6, 34.96e6),
 }
