@@ -170,6 +170,27 @@ template <typename T>
 __device__ __forceinline__ cx<T> cshfl(cx<T> z, int src) {
     return cmk<T>(__shfl(z.re, src, 64), __shfl(z.im, src, 64));
 }
+// lane K of each row of 16 lanes to the whole row (DPP row_newbcast)
+template <int K>
+__device__ __forceinline__ unsigned rowbcast_u(unsigned v) {
+    static_assert(K >= 0 && K < 16, "lane within a row of 16");
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x150 + K, 0xF, 0xF, false);
+}
+template <int K>
+__device__ __forceinline__ float rowbcast(float v) { return __uint_as_float(rowbcast_u<K>(__float_as_uint(v))); }
+template <int K>
+__device__ __forceinline__ double rowbcast(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return __longlong_as_double((long long)(((unsigned long long)rowbcast_u<K>((unsigned)(b >> 32)) << 32) |
+                                            rowbcast_u<K>((unsigned)b)));
+}
+template <int K, typename T>
+__device__ __forceinline__ cx<T> crowbcast(cx<T> z) { return cmk<T>(rowbcast<K>(z.re), rowbcast<K>(z.im)); }
+// f(integral_constant<int, J>) for each J of the sequence, in order
+template <typename F, int... J>
+__device__ __forceinline__ void static_for(F&& f, std::integer_sequence<int, J...>) {
+    (f(std::integral_constant<int, J>{}), ...);
+}
 template <typename T>
 __device__ __forceinline__ cx<T> to_t(z64 a) { return cmk<T>((T)a.re, (T)a.im); }
 template <typename T>
@@ -630,17 +651,18 @@ __device__ __forceinline__ void uinv_blocks(const cx<T>* __restrict__ S, int n, 
         const cx<T> rd = crecip(ldc<1>(S + (pr + (size_t)(k0 + min(i, kb - 1)) * n)));
 #pragma unroll
         for (int q = 0; q < 4; ++q) x[q] = i == 4 * g + q ? cmk<T>(T(1), T(0)) : czero<T>();
-#pragma unroll
-        for (int k = B - 1; k >= 0; --k) {
+        // steps k = 15 .. 0; row k of each column group comes from lane k of its row of 16
+        static_for([&](auto kc) {
+            constexpr int k = B - 1 - decltype(kc)::value;
             if (k < kb) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     if (i == k) x[q] = cmul(x[q], rd);
-                    const cx<T> xk = cshfl(x[q], 16 * g + k);
+                    const cx<T> xk = crowbcast<k>(x[q]);
                     if (i < k) cfms(x[q], u[k], xk);
                 }
             }
-        }
+        }, std::make_integer_sequence<int, B>{});
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int c = 4 * g + q;

@@ -187,12 +187,6 @@ __device__ __forceinline__ void subpanel_moves(cx<T>* __restrict__ P, int ldp, i
     __syncthreads();
 }
 
-// f(integral_constant<int, J>) for each J of the sequence, in order
-template <typename F, int... J>
-__device__ __forceinline__ void static_for(F&& f, std::integer_sequence<int, J...>) {
-    (f(std::integral_constant<int, J>{}), ...);
-}
-
 // ---- 16-column sub-panel of the LDS panel ------------------------------------------------
 // Columns [c0, c0 + sbw) of P (m x jb, ld m, logical row order), rows
 // [c0, m).  Every thread holds whole rows of the sub-panel (rows
@@ -588,21 +582,21 @@ template <typename T>
 __device__ __forceinline__ void invert_unit_lower16(const cx<T>* Q, int ldq, int kb, cx<T>* Linv,
                                                     cx<T>* __restrict__ Lg) {
     static_assert(kThreads == kSB * kSB, "one thread per entry of the block");
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x;
     const int i = tid & 15, c = tid >> 4;  // thread (i, c) holds X[i][c]
     cx<T> lrow[kSB - 1];                   // L[i][k] for k < i (zero for rows >= kb)
 #pragma unroll
     for (int k = 0; k < kSB - 1; ++k)
         lrow[k] = ldz<3>(Q + (min(i, kb - 1) + (size_t)min(k, kb - 1) * ldq), k < i && i < kb);
     cx<T> x = (i == c && c < kb) ? cmk<T>(T(1), T(0)) : czero<T>();
-#pragma unroll
-    for (int k = 0; k < kSB - 1; ++k) {
+    static_for([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
         if (k < kb - 1) {
-            // X[k][c] sits on lane k + 16 (c % 4) of this wavefront
-            const cx<T> xk = cshfl(x, (lane & 48) | k);
+            // X[k][c] sits on lane k + 16 (c % 4) of this wavefront: lane k of the row
+            const cx<T> xk = crowbcast<k>(x);
             if (i > k) cfms(x, lrow[k], xk);
         }
-    }
+    }, std::make_integer_sequence<int, kSB - 1>{});
     stc<3>(Linv + (i + c * kSB), x);
     if (i < kb && c < kb) stc<1>(Lg + (i + (size_t)c * kb), x);
 }
