@@ -166,10 +166,6 @@ __device__ __forceinline__ cx<T> cbcast(cx<T> z, int l) {
     return cmk<T>(lane_read(z.re, l), lane_read(z.im, l));
 }
 // z from lane src of the wavefront (one ds_bpermute per part).
-template <typename T>
-__device__ __forceinline__ cx<T> cshfl(cx<T> z, int src) {
-    return cmk<T>(__shfl(z.re, src, 64), __shfl(z.im, src, 64));
-}
 // lane K of each row of 16 lanes to the whole row (DPP row_newbcast)
 template <int K>
 __device__ __forceinline__ unsigned rowbcast_u(unsigned v) {
